@@ -1,0 +1,239 @@
+/*
+ * sphdem.h -- C ABI of libsphdem.so, the MI355X (gfx950) implementation of
+ * the coupled WCSPH-DEM time step.
+ *
+ * Reference counterparts are cited as file:line of hojin9908/Warp_SPH_GPT.
+ * The reference binds its path through NVIDIA Warp (wp.launch of @wp.kernel
+ * functions with SoA structs and wp.HashGrid ids); this ABI replaces that
+ * operator layer.  Python binds it with ctypes (warp_sph_gpt_amd/_lib.py).
+ *
+ * Conventions
+ *  - Every REAL buffer is void*: float when `precision` == 4 (the reference's
+ *    FP32 path) or double when `precision` == 8 (the FP64 shadow of the same
+ *    kernels).  vec3 fields are packed (N,3) arrays, as Warp's wp.vec3 arrays
+ *    and torch (N,3) tensors are.  Index/offset buffers are int32.
+ *  - The caller owns every buffer (device memory, e.g. torch tensors).  The
+ *    library allocates nothing and borrows pointers only for the work it
+ *    enqueues.  All work is enqueued on `stream` (a hipStream_t; NULL is the
+ *    default stream) in call order; no call synchronises the host.
+ *  - Every call returns SPHDEM_OK or an error code; sphdem_last_error()
+ *    returns the message of the last failure on the calling thread.
+ *  - Particle outputs are written at the particle's original index, as the
+ *    reference writes P.field[i] with i = hash_grid_point_id(grid, tid).
+ */
+#ifndef SPHDEM_H
+#define SPHDEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHDEM_OK 0
+#define SPHDEM_E_ARG 1      /* invalid argument: maps to ValueError */
+#define SPHDEM_E_HIP 2      /* HIP runtime/launch failure: RuntimeError */
+#define SPHDEM_E_OVERFLOW 3 /* CSR capacity exceeded: OverflowError */
+
+/* Solver constants of one call.  Fields mirror input/Config.py:11-36 `Solv`
+ * (h, rho0, c0, gamma, mu, g, dt, shepard_step, dem_porosity_min/max) and
+ * the cached dem_search_scales tuple (input/input_reader.py:163-168). */
+typedef struct {
+    int precision;               /* 4 = FP32, 8 = FP64 */
+    double h, support;           /* Solv.h, Solv.support = 2h */
+    double rho0, c0, gamma, mu, g, dt;
+    double eps_min, eps_max;     /* Solv.dem_porosity_min / _max */
+    int shepard_step;            /* Solv.shepard_step */
+    double radius_max;           /* search_scales[0]: max moving DEM radius */
+    double boundary_radius_max;  /* search_scales[1]: max DEM-wall radius */
+    double h_max;                /* search_scales[2]: max DEM coupling h */
+    double h_uniform;            /* search_scales[3]: common DEM h or 0 */
+} sphdem_params_t;
+
+/* input/struct.py:3-21 SPHptl */
+typedef struct {
+    void *pos, *vel, *rho_raw, *rho, *pres, *acc, *m, *flt, *porosity, *pgf, *acc_dem;
+    int n;
+} sphdem_sph_t;
+
+/* input/struct.py:23-36 BNDptl */
+typedef struct {
+    void *pos, *vel, *rho_raw, *rho, *pres, *acc, *m;
+    int n;
+} sphdem_bnd_t;
+
+/* input/struct.py:39-82 DEMptl.  CSR id/tang buffers hold `cap_*` entries;
+ * the logical edge count E is offset[n] on the device. */
+typedef struct {
+    void *pos, *vel, *acc, *force, *omega, *torque, *radius, *rho, *m, *volume,
+         *inertia, *K, *eta, *mu, *h, *flt_s, *porosity, *drag, *pressure_force;
+    int *contact_dem_offset_old, *contact_dem_id_old;
+    void *tang_dem_old;
+    int *contact_dem_offset_new, *contact_dem_id_new;
+    void *tang_dem_new;
+    int *contact_bnd_offset_old, *contact_bnd_id_old;
+    void *tang_bnd_old;
+    int *contact_bnd_offset_new, *contact_bnd_id_new;
+    void *tang_bnd_new;
+    int cap_dem_new, cap_bnd_new;   /* capacity of the *_new id/tang buffers */
+    int n;
+} sphdem_dem_t;
+
+/* input/struct.py:85-104 DEMBNDptl */
+typedef struct {
+    void *pos, *vel, *omega, *radius, *rho, *m, *volume, *inertia, *K, *eta, *mu;
+    int n;
+} sphdem_dembnd_t;
+
+/* A built hash grid: replaces wp.HashGrid (constructor main.py:117-122,
+ * build main.py:66,73,85,87) and the handle kernels receive as grid.id.
+ * Buffers are owned by the caller; sizes in brackets. */
+typedef struct {
+    int precision;        /* element type of `sorted` */
+    int dim_x, dim_y, dim_z;
+    double cell_width;    /* the `radius` of the last build */
+    int n;                /* points in the last build */
+    int *point_cells;     /* [n]   bucket key of original point i */
+    int *point_ids;       /* [n]   sorted slot -> original index (stable) */
+    int *rank;            /* [n]   scratch */
+    int *cell_lo;         /* [ncell] first sorted slot of each bucket */
+    int *cell_hi;         /* [ncell] one past the last slot of each bucket */
+    void *sorted;         /* [n*4] (x, y, z, w) in sorted order, w from `w` */
+    int *scan_tmp;        /* [sphdem_scan_scratch(ncell)] */
+} sphdem_grid_t;
+
+/* Per-step working arrays of the fused step, in grid-sorted order.
+ * Sizes: fluid arrays >= fluid count, wall arrays >= wall count. */
+typedef struct {
+    void *fl_rho, *fl_pres, *fl_V;  /* [nf] density, pressure, m/rho */
+    void *fl_vel, *fl_pgf;          /* [nf*4] (vel, 0), (-grad p, rho) */
+    void *wl_rho, *wl_pres;         /* [nb] */
+    void *wl_vel;                   /* [nb*4] */
+    int *scan_tmp;                  /* [sphdem_scan_scratch(n_dem + 1)] */
+    int *flags;                     /* [4] bit 0: DEM CSR overflow */
+    int cap_f, cap_b;
+} sphdem_work_t;
+
+/* Library identity and diagnostics. */
+int sphdem_version(void);
+const char *sphdem_last_error(void);
+int sphdem_device_count(void);
+
+/* Scratch ints an exclusive scan of n entries needs. */
+int sphdem_scan_scratch(int n);
+
+/* Exclusive prefix sum of a[0..n) in place: wp.utils.array_scan(a, a,
+ * inclusive=False), source/Simulation.py:148-149, 181-182. */
+int sphdem_exclusive_scan(int *a, int n, int *scratch, void *stream);
+
+/* wp.HashGrid.build(points, radius) (main.py:66,73,85,87): bucket keys with
+ * 1/radius cells folded into dim_x*dim_y*dim_z buckets, a stable counting
+ * sort of the points by key, bucket ranges, and a sorted copy of the points
+ * (w column from `w`, or 0 when w is NULL).  `points`/`w` use g->precision. */
+int sphdem_grid_build(sphdem_grid_t *g, const void *points, int n, double radius,
+                      const void *w, void *stream);
+
+/* Refill the w column of the sorted copy from w[original index] (the fluid
+ * and wall kernels read neighbour masses from it). */
+int sphdem_grid_set_w(sphdem_grid_t *g, const void *w, void *stream);
+
+/* Warp-convention bucket bounds (0,0 for empty buckets) for parity views. */
+int sphdem_grid_warp_bounds(const sphdem_grid_t *g, int *cell_starts, int *cell_ends,
+                            void *stream);
+
+/* Number of true neighbours (r^2 < radius^2, self included) of each subject
+ * point of grid gs in grid gn, written at the subject's original index. */
+int sphdem_neighbour_counts(const sphdem_grid_t *gs, const sphdem_grid_t *gn,
+                            double radius, int *counts, void *stream);
+
+/* kernel/Kernel_KNL.py:7-41 sampled at r[0..n): W and dW/dr. */
+int sphdem_wendland(int precision, const void *r, int n, double h, void *w, void *dw,
+                    void *stream);
+
+/* kernel/KERNEL_SPHDEM_interaction.py:11-40 DEM_drag_beta sampled elementwise. */
+int sphdem_drag_beta(int precision, const void *eps, const void *rho, const void *mu,
+                     const void *diameter, const void *speed, int n, void *beta,
+                     void *stream);
+
+/* ---- one entry per reference kernel ------------------------------------ */
+/* kernel/KERNEL_rho.py:6-40 */
+int sphdem_shepard_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream);
+/* kernel/KERNEL_rho.py:45-82 */
+int sphdem_density_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream);
+/* kernel/KERNEL_rho.py:87-126 */
+int sphdem_density_bnd(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream);
+/* kernel/KERNEL_pres.py:6-23 and 26-45: p = B*max((rho/rho0)^gamma - 1, 0) */
+int sphdem_pres(const sphdem_params_t *p, const void *rho, void *pres, int n, void *stream);
+/* kernel/KERNEL_force.py:7-71 (uses w as scratch) */
+int sphdem_force_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                     const sphdem_grid_t *gs, const sphdem_grid_t *gb,
+                     const sphdem_work_t *w, void *stream);
+/* kernel/KERNEL_step.py:6-22 */
+int sphdem_step_sph(const sphdem_params_t *p, sphdem_sph_t *P, void *stream);
+/* kernel/KERNEL_DEM_force.py:9-22 */
+int sphdem_count_dem_contacts(const sphdem_params_t *p, sphdem_dem_t *D,
+                              const sphdem_grid_t *gd, double radius_max, void *stream);
+/* kernel/KERNEL_DEM_force.py:77-135 (flags[0] bit 0 set on capacity overflow) */
+int sphdem_force_dem(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_grid_t *gd,
+                     double radius_max, double g, int *flags, void *stream);
+/* kernel/KERNEL_DEM_BC.py:10-24 */
+int sphdem_count_bnd_contacts(const sphdem_params_t *p, sphdem_dem_t *D,
+                              const sphdem_dembnd_t *W, const sphdem_grid_t *gw,
+                              double radius_max, void *stream);
+/* kernel/KERNEL_DEM_BC.py:27-84 */
+int sphdem_bc_dem(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_dembnd_t *W,
+                  const sphdem_grid_t *gw, double radius_max, int *flags, void *stream);
+/* kernel/KERNEL_DEM_step.py:6-30 */
+int sphdem_step_dem(const sphdem_params_t *p, sphdem_dem_t *D, void *stream);
+/* kernel/KERNEL_SPHDEM_interaction.py:43-115; `support`/`h` are the coupling
+ * support (2*h_max) and the uniform DEM h (0 when heterogeneous). */
+int sphdem_prep_sphdem(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       sphdem_dem_t *D, const sphdem_grid_t *gs, const sphdem_grid_t *gb,
+                       const sphdem_grid_t *gd, double support, double h,
+                       const sphdem_work_t *w, void *stream);
+/* kernel/KERNEL_SPHDEM_interaction.py:118-187 */
+int sphdem_interaction_dem(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_dem_t *D,
+                           const sphdem_grid_t *gs, const sphdem_grid_t *gd,
+                           const sphdem_work_t *w, void *stream);
+/* kernel/KERNEL_SPHDEM_interaction.py:190-226 */
+int sphdem_interaction_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_dem_t *D,
+                           const sphdem_grid_t *gs, const sphdem_grid_t *gd,
+                           double support, void *stream);
+
+/* ---- fused steps (the hot path) ---------------------------------------- */
+/* source/Simulation.py:17-99 SPH_OneStep: Shepard (at step 0 and every
+ * shepard_step), density and EOS of fluid and wall, force, and (integrate)
+ * semi-implicit Euler.  Grids must be built from the current positions. */
+int sphdem_sph_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                    const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step,
+                    int integrate, const sphdem_work_t *w, void *stream);
+
+/* source/Simulation.py:102-221 SPHDEM_OneStep, split at the two points where
+ * the reference reads the CSR totals back to the host:
+ *   stage 0: SPH fields and forces, DEM-DEM counts + scan (E_dem = offset[n])
+ *   stage 1: DEM-DEM fill, DEM-wall counts + scan
+ *   stage 2: DEM-wall fill, coupling, integration of SPH and DEM
+ * The caller swaps old/new CSR sides after stages 1 and 2 exactly as the
+ * reference does.  stage -1 runs all three with no host round trip; it needs
+ * *_new capacities large enough (flags bit 0 reports overflow) and copies the
+ * new CSR back into the old buffers instead of swapping. */
+int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                        sphdem_dem_t *D, sphdem_dembnd_t *W, const sphdem_grid_t *gs,
+                        const sphdem_grid_t *gb, const sphdem_grid_t *gd,
+                        const sphdem_grid_t *gw, int step, int stage,
+                        const sphdem_work_t *w, void *stream);
+
+/* ---- FP64 diagnostics (no reference counterpart) ----------------------- */
+/* out[0] = total fluid mass, out[1] = fluid KE 1/2 sum m|v|^2,
+ * out[2] = DEM KE incl. 1/2 I|w|^2, out[3] = DEM mass, out[4] = surge front
+ * (max fluid x), all accumulated in FP64.  `out` is device memory [5]. */
+int sphdem_reduce_f64(int precision, const void *sph_pos, const void *sph_vel,
+                      const void *sph_m, int nf, const void *dem_vel, const void *dem_omega,
+                      const void *dem_m, const void *dem_inertia, int nd, double *out,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

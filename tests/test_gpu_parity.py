@@ -1,0 +1,289 @@
+"""HIP path against the CPU oracle on the same seeded inputs.
+
+Bars (stated per test):
+  * bit-exact: bucket keys, sorted order, bucket bounds, neighbour counts,
+    contact CSR offsets and ids;
+  * FP32 fields: relative error (max-norm over particles, rows as vectors)
+    <= 1e-5 against the FP32 oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import COMPACT, assert_rel, generation, np_, particles, rel_err, solv_for, to_oracle
+from oracle import oracle as O
+from warp_sph_gpt_amd import kernels as K
+from warp_sph_gpt_amd.hashgrid import HashGrid, neighbour_counts
+from warp_sph_gpt_amd.simulation import SPH_OneStep, SPHDEM_OneStep
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TOL = 1.0e-5
+
+
+def gpu(structs):
+    return [s.to(DEV) for s in structs]
+
+
+def wet_scene(seed=0, **kw):
+    """C0 dam break (dx=0.02) with the 20 DEM moved into the fluid, one
+    overlapping pair, one sphere on the floor spheres, and seeded random
+    velocities, so that every kernel has non-trivial work."""
+    gen = generation(**kw)
+    P, B, D, W = particles(gen)
+    rng = np.random.default_rng(seed)
+    dpos = np_(D.pos).astype(np.float64)
+    dpos[:, 2] -= 0.32
+    dpos[1] = dpos[0] + [0.049, 0.0, 0.0]
+    dpos[19] = [0.45, 0.2, 0.02]
+    D.pos = torch.tensor(dpos, dtype=torch.float32)
+    D.vel = torch.tensor(rng.normal(0, 0.05, (D.n, 3)), dtype=torch.float32)
+    D.omega = torch.tensor(rng.normal(0, 0.5, (D.n, 3)), dtype=torch.float32)
+    P.vel = torch.tensor(rng.normal(0, 0.05, (P.n, 3)), dtype=torch.float32)
+    P.pos = P.pos + torch.tensor(rng.uniform(-0.1, 0.1, (P.n, 3)) * gen.dx, dtype=torch.float32)
+    return gen, [P, B, D, W]
+
+
+# ---------------------------------------------------------------- grid
+
+
+def _grid_pair(points: np.ndarray, radius: float, dims):
+    og = O.OracleGrid(*dims).build(points, radius)
+    gg = HashGrid(*dims, device=DEV).build(torch.tensor(points, device=DEV), radius)
+    return og, gg
+
+
+@pytest.mark.parametrize("case", ["fluid", "wall", "folded", "dem"])
+def test_grid_keys_order_bounds_bit_exact(case):
+    gen, (P, B, D, W) = wet_scene()
+    rng = np.random.default_rng(1)
+    if case == "fluid":
+        pts, r, dims = np_(P.pos), gen.support, (64, 64, 64)
+    elif case == "wall":
+        pts, r, dims = np_(B.pos), gen.support, (64, 64, 64)
+    elif case == "folded":  # 40 cells per axis folded into 8: aliasing and negatives
+        pts = rng.uniform(-1.0, 1.0, (5000, 3)).astype(np.float32)
+        r, dims = 0.05, (8, 8, 8)
+    else:
+        pts, r, dims = np_(D.pos), 0.16, (64, 64, 64)
+    og, gg = _grid_pair(pts, r, dims)
+    np.testing.assert_array_equal(np_(gg.point_cells), og.point_cells[np.argsort(og.point_ids)])
+    np.testing.assert_array_equal(np_(gg.point_ids), og.point_ids)
+    s, e = gg.warp_bounds()
+    np.testing.assert_array_equal(np_(s), og.cell_starts)
+    np.testing.assert_array_equal(np_(e), og.cell_ends)
+    np.testing.assert_array_equal(np_(gg.sorted_points)[:, :3], pts[og.point_ids])
+
+
+def test_neighbour_counts_bit_exact():
+    gen, (P, B, D, W) = wet_scene()
+    for sp, npos, r in ((np_(P.pos), np_(P.pos), gen.support), (np_(P.pos), np_(B.pos), gen.support),
+                        (np_(B.pos), np_(P.pos), gen.support), (np_(P.pos), np_(P.pos), 0.16)):
+        os_, gs_ = _grid_pair(sp, gen.support, (64, 64, 64))
+        on, gn = _grid_pair(npos, gen.support, (64, 64, 64))
+        ref = O.neighbour_counts(os_, sp, on, npos, r)
+        got = np_(neighbour_counts(gs_, gn, r))
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_exclusive_scan_matches_prefix_sum():
+    rng = np.random.default_rng(2)
+    for n in (1, 7, 2048, 2049, 70000, 300001):
+        a = rng.integers(0, 50, n).astype(np.int32)
+        t = torch.tensor(a, device=DEV)
+        K.array_scan(t, t, inclusive=False)
+        ref = np.concatenate([[0], np.cumsum(a)[:-1]]).astype(np.int32)
+        np.testing.assert_array_equal(np_(t), ref)
+
+
+# ---------------------------------------------------------------- single kernels
+
+
+class Scene:
+    """The wet scene on both sides with all four grids built."""
+
+    def __init__(self, seed=0, dtype=np.float32, **kw):
+        gen, structs = wet_scene(seed, **kw)
+        self.gen = gen
+        self.solv = solv_for(gen)
+        self.o = to_oracle(structs, dtype)
+        tdt = torch.float32 if dtype == np.float32 else torch.float64
+        self.g = [s.to(DEV, tdt) for s in structs]
+        P, B, D, W = self.o
+        self.scales = O.dem_search_scales(D, W)
+        radii = (gen.support, gen.support, max(2 * self.scales[0], 2 * self.scales[2]),
+                 max(2 * self.scales[1], gen.support))
+        gsl = gen.grid_slice
+        self.og = [O.OracleGrid(gsl, gsl, gsl, dtype).build(s.pos, r) for s, r in zip(self.o, radii)]
+        self.gg = [HashGrid(gsl, gsl, gsl, device=DEV).build(s.pos, r)
+                   for s, r in zip(self.g, radii)]
+
+    def compare(self, names, kinds=(0,), tol=TOL, floor=0.0):
+        errs = {}
+        for k in kinds:
+            for name in names:
+                a = np_(getattr(self.g[k], name))
+                b = getattr(self.o[k], name)
+                errs[(k, name)] = assert_rel(a, b, tol, floor, f"struct {k} field {name}")
+        return errs
+
+
+def test_density_pressure_force_kernels():
+    S = Scene()
+    s = S.solv
+    (P, B, D, W), (gP, gB, gD, gW) = S.o, S.g
+    args_o, args_g = (P, B, *S.og[:2]), (gP, gB, *S.gg[:2])
+    O.Kernel_shepard_sph(*args_o, s.support, s.h)
+    K.Kernel_shepard_sph(*args_g, s.support, s.h)
+    S.compare(["flt"])
+    O.Kernel_density_sph(*args_o, s.support, s.h)
+    K.Kernel_density_sph(*args_g, s.support, s.h)
+    O.Kernel_density_bnd(*args_o, s.support, s.h)
+    K.Kernel_density_bnd(*args_g, s.support, s.h)
+    S.compare(["rho"], kinds=(0, 1))
+    for st_o, st_g in ((P, gP), (B, gB)):
+        O.Kernel_pres_sph(st_o, s.rho0, s.c0, s.gamma)
+        K.Kernel_pres_sph(st_g, s.rho0, s.c0, s.gamma)
+    # pressure is B*((rho/rho0)^7 - 1): near rho0 it is rounding noise of B,
+    # so its error is measured against B (tait stiffness) as the floor
+    stiff = s.rho0 * s.c0 ** 2 / s.gamma
+    S.compare(["pres"], kinds=(0, 1), floor=1e-2 * stiff)
+    O.Kernel_force_sph(*args_o, s.support, s.h, s.mu, s.g)
+    K.Kernel_force_sph(*args_g, s.support, s.h, s.mu, s.g)
+    S.compare(["acc"])
+
+
+def test_dem_contact_kernels_and_csr():
+    S = Scene()
+    s = S.solv
+    (P, B, D, W), (gP, gB, gD, gW) = S.o, S.g
+    rmax, rbmax = S.scales[0], S.scales[1]
+    E = O.run_force_dem(D, S.og[2], rmax, s.g, s.dt)
+    # GPU, reference-style lifecycle through the per-kernel entries
+    gD.contact_dem_offset_new.zero_()
+    K.Kernel_count_dem_contacts(gD, S.gg[2], rmax)
+    K.array_scan(gD.contact_dem_offset_new, gD.contact_dem_offset_new)
+    Eg = int(gD.contact_dem_offset_new[-1].item())
+    assert Eg == E and E > 0
+    gD.contact_dem_id_new = torch.empty(Eg, dtype=torch.int32, device=DEV)
+    gD.tang_dem_new = torch.empty((Eg, 3), device=DEV)
+    K.Kernel_force_dem(gD, S.gg[2], rmax, s.g, s.dt)
+    np.testing.assert_array_equal(np_(gD.contact_dem_offset_new), D.contact_dem_offset_old)
+    np.testing.assert_array_equal(np_(gD.contact_dem_id_new), D.contact_dem_id_old)
+    assert_rel(np_(gD.tang_dem_new), D.tang_dem_old, TOL, what="tang_dem")
+    S.compare(["force", "torque"], kinds=(2,))
+    Eb = O.run_bc_dem(D, W, S.og[3], rbmax, s.dt)
+    gD.contact_bnd_offset_new.zero_()
+    K.Kernel_count_bnd_contacts(gD, gW, S.gg[3], rbmax)
+    K.array_scan(gD.contact_bnd_offset_new, gD.contact_bnd_offset_new)
+    assert int(gD.contact_bnd_offset_new[-1].item()) == Eb and Eb > 0
+    gD.contact_bnd_id_new = torch.empty(Eb, dtype=torch.int32, device=DEV)
+    gD.tang_bnd_new = torch.empty((Eb, 3), device=DEV)
+    K.Kernel_bc_dem(gD, gW, S.gg[3], rbmax, s.dt)
+    np.testing.assert_array_equal(np_(gD.contact_bnd_id_new), D.contact_bnd_id_old)
+    assert_rel(np_(gD.tang_bnd_new), D.tang_bnd_old, TOL, what="tang_bnd")
+    S.compare(["force", "torque"], kinds=(2,))
+    O.Kernel_step_dem(D, s.dt)
+    K.Kernel_step_dem(gD, s.dt)
+    S.compare(["pos", "vel", "omega", "acc"], kinds=(2,))
+
+
+@pytest.mark.parametrize("uniform_h", [True, False])
+def test_coupling_kernels(uniform_h):
+    S = Scene()
+    s = S.solv
+    (P, B, D, W), (gP, gB, gD, gW) = S.o, S.g
+    if not uniform_h:
+        hs = np.linspace(0.05, 0.08, D.n).astype(np.float32)
+        D.h[:] = hs
+        gD.h.copy_(torch.tensor(hs))
+        S.scales = O.dem_search_scales(D, W)
+    rmax, rbmax, hmax, hu = S.scales
+    # a pressure field that is not rounding noise
+    pres = (2000.0 * (0.5 - P.pos[:, 2])).astype(np.float32)
+    P.pres[:] = pres
+    gP.pres.copy_(torch.tensor(pres))
+    bp = (2000.0 * (0.5 - B.pos[:, 2])).astype(np.float32)
+    B.pres[:] = bp
+    gB.pres.copy_(torch.tensor(bp))
+    O.Kernel_prep_sphdem(P, B, D, *S.og[:3], 2 * hmax, hu, s.support, s.h, s.dem_porosity_min,
+                         s.dem_porosity_max)
+    K.Kernel_prep_sphdem(gP, gB, gD, *S.gg[:3], 2 * hmax, hu, s.support, s.h,
+                         s.dem_porosity_min, s.dem_porosity_max)
+    S.compare(["porosity", "pgf"])
+    assert P.porosity.min() < 0.99
+    O.Kernel_interaction_dem(P, D, S.og[0], S.og[2], s.mu, s.dem_porosity_min,
+                             s.dem_porosity_max, s.dt)
+    K.Kernel_interaction_dem(gP, gD, S.gg[0], S.gg[2], s.mu, s.dem_porosity_min,
+                             s.dem_porosity_max, s.dt)
+    S.compare(["flt_s", "porosity", "drag", "pressure_force", "force"], kinds=(2,))
+    O.Kernel_interaction_sph(P, D, S.og[0], S.og[2], 2 * hmax)
+    K.Kernel_interaction_sph(gP, gD, S.gg[0], S.gg[2], 2 * hmax)
+    S.compare(["acc_dem", "acc"])
+    O.Kernel_step_sph(P, s.dt)
+    K.Kernel_step_sph(gP, s.dt)
+    S.compare(["pos", "vel"])
+
+
+def test_oracle_prep_skip_changes_nothing():
+    """The oracle's skip of the coupling-scale fluid sum where no DEM is
+    within 2h_b leaves porosity and -grad p bit-identical."""
+    S = Scene()
+    s = S.solv
+    P, B, D, W = S.o
+    P2 = P.copy()
+    hmax, hu = S.scales[2], S.scales[3]
+    O.Kernel_prep_sphdem(P, B, D, *S.og[:3], 2 * hmax, hu, s.support, s.h, 0.05, 1.0, 1)
+    O.Kernel_prep_sphdem(P2, B, D, *S.og[:3], 2 * hmax, hu, s.support, s.h, 0.05, 1.0, 0)
+    np.testing.assert_array_equal(P.porosity, P2.porosity)
+    np.testing.assert_array_equal(P.pgf, P2.pgf)
+
+
+# ---------------------------------------------------------------- whole steps
+
+
+def _build_moving(S, gsl):
+    s = S.solv
+    S.og[0].build(S.o[0].pos, s.support)
+    S.gg[0].build(S.g[0].pos, s.support)
+    r = max(2 * S.scales[0], 2 * S.scales[2])
+    S.og[2].build(S.o[2].pos, r)
+    S.gg[2].build(S.g[2].pos, r)
+
+
+def test_sph_only_steps_match_oracle():
+    S = Scene()
+    s = S.solv
+    P, B = S.o[:2]
+    gP, gB = S.g[:2]
+    for step in range(25):  # crosses the Shepard refresh at step 20
+        S.og[0].build(P.pos, s.support)
+        S.gg[0].build(gP.pos, s.support)
+        O.SPH_OneStep(s, P, B, S.og[0], S.og[1], step)
+        SPH_OneStep(s, gP, gB, S.gg[0], S.gg[1], step)
+    stiff = s.rho0 * s.c0 ** 2 / s.gamma
+    S.compare(["pos", "vel", "rho", "flt", "acc"], tol=5 * TOL)
+    S.compare(["pres"], floor=1e-2 * stiff, tol=5 * TOL)
+
+
+def test_coupled_steps_match_oracle_and_csr_bit_exact():
+    S = Scene()
+    s = S.solv
+    P, B, D, W = S.o
+    gP, gB, gD, gW = S.g
+    for step in range(5):
+        _build_moving(S, s.grid_slice)
+        O.SPHDEM_OneStep(s, P, B, D, W, *S.og, step, S.scales)
+        SPHDEM_OneStep(s, gP, gB, gD, gW, *S.gg, step, S.scales)
+        for kind in ("dem", "bnd"):
+            np.testing.assert_array_equal(np_(getattr(gD, f"contact_{kind}_offset_old")),
+                                          getattr(D, f"contact_{kind}_offset_old"))
+            np.testing.assert_array_equal(np_(getattr(gD, f"contact_{kind}_id_old")),
+                                          getattr(D, f"contact_{kind}_id_old"))
+    S.compare(["pos", "vel", "rho", "acc", "porosity", "pgf", "acc_dem"], tol=5 * TOL)
+    S.compare(["pos", "vel", "omega", "force", "torque", "drag", "pressure_force", "flt_s"],
+              kinds=(2,), tol=5 * TOL)
+    assert_rel(np_(gD.tang_dem_old), D.tang_dem_old, 5 * TOL, what="tang_dem")
+    assert_rel(np_(gD.tang_bnd_old), D.tang_bnd_old, 5 * TOL, what="tang_bnd")

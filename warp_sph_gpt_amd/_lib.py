@@ -1,0 +1,252 @@
+"""ctypes binding of libsphdem.so (declared in include/sphdem.h).
+
+Structures below mirror the C structs field for field.  Tensors are passed
+as raw device pointers; every pointer is checked for device, dtype and
+contiguity before a call.  Error codes map to the reference's exception
+types: SPHDEM_E_ARG -> ValueError, SPHDEM_E_OVERFLOW -> OverflowError,
+SPHDEM_E_HIP -> RuntimeError.  There is no fallback: if the library is not
+built, or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_void_p
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libsphdem.so"
+SPHDEM_OK, SPHDEM_E_ARG, SPHDEM_E_HIP, SPHDEM_E_OVERFLOW = 0, 1, 2, 3
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("precision", c_int), ("h", c_double), ("support", c_double),
+                ("rho0", c_double), ("c0", c_double), ("gamma", c_double), ("mu", c_double),
+                ("g", c_double), ("dt", c_double), ("eps_min", c_double), ("eps_max", c_double),
+                ("shepard_step", c_int), ("radius_max", c_double),
+                ("boundary_radius_max", c_double), ("h_max", c_double), ("h_uniform", c_double)]
+
+
+def _ptr_fields(names):
+    return [(n, c_void_p) for n in names]
+
+
+class SphView(ctypes.Structure):
+    _fields_ = _ptr_fields(("pos", "vel", "rho_raw", "rho", "pres", "acc", "m", "flt",
+                            "porosity", "pgf", "acc_dem")) + [("n", c_int)]
+
+
+class BndView(ctypes.Structure):
+    _fields_ = _ptr_fields(("pos", "vel", "rho_raw", "rho", "pres", "acc", "m")) + [("n", c_int)]
+
+
+DEM_REAL = ("pos", "vel", "acc", "force", "omega", "torque", "radius", "rho", "m", "volume",
+            "inertia", "K", "eta", "mu", "h", "flt_s", "porosity", "drag", "pressure_force")
+DEM_CSR = ("contact_dem_offset_old", "contact_dem_id_old", "tang_dem_old",
+           "contact_dem_offset_new", "contact_dem_id_new", "tang_dem_new",
+           "contact_bnd_offset_old", "contact_bnd_id_old", "tang_bnd_old",
+           "contact_bnd_offset_new", "contact_bnd_id_new", "tang_bnd_new")
+
+
+class DemView(ctypes.Structure):
+    _fields_ = (_ptr_fields(DEM_REAL) + _ptr_fields(DEM_CSR)
+                + [("cap_dem_new", c_int), ("cap_bnd_new", c_int), ("n", c_int)])
+
+
+class DemBndView(ctypes.Structure):
+    _fields_ = _ptr_fields(("pos", "vel", "omega", "radius", "rho", "m", "volume", "inertia",
+                            "K", "eta", "mu")) + [("n", c_int)]
+
+
+class GridStruct(ctypes.Structure):
+    _fields_ = [("precision", c_int), ("dim_x", c_int), ("dim_y", c_int), ("dim_z", c_int),
+                ("cell_width", c_double), ("n", c_int), ("point_cells", c_void_p),
+                ("point_ids", c_void_p), ("rank", c_void_p), ("cell_lo", c_void_p),
+                ("cell_hi", c_void_p), ("sorted", c_void_p), ("scan_tmp", c_void_p)]
+
+
+class WorkStruct(ctypes.Structure):
+    _fields_ = _ptr_fields(("fl_rho", "fl_pres", "fl_V", "fl_vel", "fl_pgf", "wl_rho",
+                            "wl_pres", "wl_vel", "scan_tmp", "flags")) + [
+        ("cap_f", c_int), ("cap_b", c_int)]
+
+
+_SIGNATURES = {
+    "sphdem_version": ([], c_int),
+    "sphdem_last_error": ([], ctypes.c_char_p),
+    "sphdem_device_count": ([], c_int),
+    "sphdem_scan_scratch": ([c_int], c_int),
+    "sphdem_exclusive_scan": ([c_void_p, c_int, c_void_p, c_void_p], c_int),
+    "sphdem_grid_build": ([POINTER(GridStruct), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
+    "sphdem_grid_set_w": ([POINTER(GridStruct), c_void_p, c_void_p], c_int),
+    "sphdem_grid_warp_bounds": ([POINTER(GridStruct), c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_neighbour_counts": ([POINTER(GridStruct), POINTER(GridStruct), c_double, c_void_p,
+                                 c_void_p], c_int),
+    "sphdem_wendland": ([c_int, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_drag_beta": ([c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                          c_void_p, c_void_p], c_int),
+    "sphdem_shepard_sph": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                            POINTER(GridStruct), POINTER(GridStruct), c_void_p], c_int),
+    "sphdem_density_sph": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                            POINTER(GridStruct), POINTER(GridStruct), c_void_p], c_int),
+    "sphdem_density_bnd": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                            POINTER(GridStruct), POINTER(GridStruct), c_void_p], c_int),
+    "sphdem_pres": ([POINTER(Params), c_void_p, c_void_p, c_int, c_void_p], c_int),
+    "sphdem_force_sph": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                          POINTER(GridStruct), POINTER(GridStruct), POINTER(WorkStruct),
+                          c_void_p], c_int),
+    "sphdem_step_sph": ([POINTER(Params), POINTER(SphView), c_void_p], c_int),
+    "sphdem_count_dem_contacts": ([POINTER(Params), POINTER(DemView), POINTER(GridStruct),
+                                   c_double, c_void_p], c_int),
+    "sphdem_force_dem": ([POINTER(Params), POINTER(DemView), POINTER(GridStruct), c_double,
+                          c_double, c_void_p, c_void_p], c_int),
+    "sphdem_count_bnd_contacts": ([POINTER(Params), POINTER(DemView), POINTER(DemBndView),
+                                   POINTER(GridStruct), c_double, c_void_p], c_int),
+    "sphdem_bc_dem": ([POINTER(Params), POINTER(DemView), POINTER(DemBndView),
+                       POINTER(GridStruct), c_double, c_void_p, c_void_p], c_int),
+    "sphdem_step_dem": ([POINTER(Params), POINTER(DemView), c_void_p], c_int),
+    "sphdem_prep_sphdem": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                            POINTER(DemView), POINTER(GridStruct), POINTER(GridStruct),
+                            POINTER(GridStruct), c_double, c_double, POINTER(WorkStruct),
+                            c_void_p], c_int),
+    "sphdem_interaction_dem": ([POINTER(Params), POINTER(SphView), POINTER(DemView),
+                                POINTER(GridStruct), POINTER(GridStruct), POINTER(WorkStruct),
+                                c_void_p], c_int),
+    "sphdem_interaction_sph": ([POINTER(Params), POINTER(SphView), POINTER(DemView),
+                                POINTER(GridStruct), POINTER(GridStruct), c_double, c_void_p],
+                               c_int),
+    "sphdem_sph_step": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                         POINTER(GridStruct), POINTER(GridStruct), c_int, c_int,
+                         POINTER(WorkStruct), c_void_p], c_int),
+    "sphdem_coupled_step": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                             POINTER(DemView), POINTER(DemBndView), POINTER(GridStruct),
+                             POINTER(GridStruct), POINTER(GridStruct), POINTER(GridStruct),
+                             c_int, c_int, POINTER(WorkStruct), c_void_p], c_int),
+    "sphdem_reduce_f64": ([c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                           c_void_p, c_void_p, c_int, c_void_p, c_void_p], c_int),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libsphdem.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m warp_sph_gpt_amd.build`")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> tuple:
+    return tuple(_SIGNATURES)
+
+
+def check(rc: int, what: str) -> None:
+    if rc == SPHDEM_OK:
+        return
+    msg = load().sphdem_last_error().decode(errors="replace")
+    text = f"{what}: {msg}"
+    if rc == SPHDEM_E_ARG:
+        raise ValueError(text)
+    if rc == SPHDEM_E_OVERFLOW:
+        raise OverflowError(text)
+    raise RuntimeError(text)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)
+
+
+def stream_handle():
+    return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def precision_of(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return 4
+    if dtype == torch.float64:
+        return 8
+    raise ValueError(f"particle arrays must be float32 or float64, not {dtype}")
+
+
+def ptr(t, dtype=None, what="array"):
+    """Device pointer of a contiguous CUDA tensor (None for missing/empty)."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch tensor")
+    if t.numel() == 0:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"{what} must live on a GPU device (got {t.device}); "
+                         "the SPH-DEM kernels run only on the GPU")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{what} must be {dtype}, not {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    return t.data_ptr()
+
+
+def _real_dtype(struct):
+    return struct.pos.dtype
+
+
+def sph_view(P) -> SphView:
+    dt = _real_dtype(P)
+    v = SphView()
+    for name, _ in P.FIELDS:
+        setattr(v, name, ptr(getattr(P, name), dt, f"P_sph.{name}"))
+    v.n = P.n
+    return v
+
+
+def bnd_view(B) -> BndView:
+    dt = _real_dtype(B)
+    v = BndView()
+    for name, _ in B.FIELDS:
+        setattr(v, name, ptr(getattr(B, name), dt, f"P_bnd.{name}"))
+    v.n = B.n
+    return v
+
+
+def dem_view(D) -> DemView:
+    dt = _real_dtype(D)
+    v = DemView()
+    for name in DEM_REAL:
+        setattr(v, name, ptr(getattr(D, name), dt, f"P_dem.{name}"))
+    for name in DEM_CSR:
+        t = getattr(D, name)
+        want = dt if name.startswith("tang_") else torch.int32
+        setattr(v, name, ptr(t, want, f"P_dem.{name}"))
+    v.cap_dem_new = int(D.contact_dem_id_new.shape[0])
+    v.cap_bnd_new = int(D.contact_bnd_id_new.shape[0])
+    if D.tang_dem_new.shape[0] < v.cap_dem_new:
+        v.cap_dem_new = int(D.tang_dem_new.shape[0])
+    if D.tang_bnd_new.shape[0] < v.cap_bnd_new:
+        v.cap_bnd_new = int(D.tang_bnd_new.shape[0])
+    v.n = D.n
+    return v
+
+
+def dembnd_view(W) -> DemBndView:
+    dt = _real_dtype(W)
+    v = DemBndView()
+    for name, _ in W.FIELDS:
+        setattr(v, name, ptr(getattr(W, name), dt, f"P_dem_bnd.{name}"))
+    v.n = W.n
+    return v
+
+
+def params(precision: int, **kw) -> Params:
+    p = Params()
+    p.precision = precision
+    for name, value in kw.items():
+        setattr(p, name, value)
+    return p

@@ -1,0 +1,53 @@
+"""Build recipe for libsphdem.so (hand-written HIP for gfx950).
+
+Run as ``python -m warp_sph_gpt_amd.build``.  The library is built in-tree
+next to this file so that it travels with the repository to the GPU host.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libsphdem.so"
+SOURCES = ("grid.hip", "sph.hip", "dem.hip")
+
+# -ffp-contract=off: every a*b+c rounds twice, as the reference's FP32
+# expressions are written.  Packed FP32 instructions are disabled.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    path = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(path).exists():
+        raise RuntimeError("hipcc not found; ROCm is required to build libsphdem.so")
+    return path
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every HIP source into warp_sph_gpt_amd/libsphdem.so."""
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", str(tmp), *[str(CSRC / s) for s in SOURCES]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+    if verbose and res.stderr.strip():
+        sys.stderr.write(res.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

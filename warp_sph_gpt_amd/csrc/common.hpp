@@ -1,0 +1,234 @@
+// common.hpp -- device types and helpers shared by the libsphdem kernels.
+//
+// All kernels are templates over the real type T (float: the reference's
+// FP32 path; double: the FP64 shadow).  The library is compiled with
+// -ffp-contract=off so every a*b+c below rounds twice, as the reference's
+// expressions are written; divisions and square roots are the correctly
+// rounded IEEE operations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/sphdem.h"
+
+namespace sphdem {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int check_launch(const char *what);
+
+#define SPHDEM_REQUIRE(cond, msg)                                   \
+    do {                                                            \
+        if (!(cond)) return ::sphdem::fail(SPHDEM_E_ARG, (msg));    \
+    } while (0)
+
+// Device-wide exclusive scan (grid.hip); `scratch` holds scan_scratch(n) ints.
+int scan_scratch(int n);
+int exclusive_scan(const int *in, int *out, int n, int *scratch, hipStream_t st);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline int blocks_for(long n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// ---------------------------------------------------------------- vectors
+// Overloads that keep float math in float on both compilation passes.
+__device__ __forceinline__ float msqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double msqrt(double x) { return ::sqrt(x); }
+// FP32 pow is evaluated in double and rounded once to float: the correctly
+// rounded result (up to a double-rounding tie), the same value the CPU oracle
+// forms, so the EOS and drag laws agree bit for bit across the two.
+__device__ __forceinline__ float mpow(float a, float b) {
+    return (float)::pow((double)a, (double)b);
+}
+__device__ __forceinline__ double mpow(double a, double b) { return ::pow(a, b); }
+__device__ __forceinline__ float mabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double mabs(double x) { return ::fabs(x); }
+
+template <typename T> struct alignas(sizeof(T) * 4) V4 { T x, y, z, w; };
+template <typename T> struct V3 { T x, y, z; };
+
+template <typename T> __device__ __forceinline__ V3<T> mk3(T x, T y, T z) { return {x, y, z}; }
+template <typename T> __device__ __forceinline__ V3<T> ld3(const T *a, int i) {
+    return {a[3 * i], a[3 * i + 1], a[3 * i + 2]};
+}
+template <typename T> __device__ __forceinline__ void st3(T *a, int i, V3<T> v) {
+    a[3 * i] = v.x; a[3 * i + 1] = v.y; a[3 * i + 2] = v.z;
+}
+template <typename T> __device__ __forceinline__ V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator-(V3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <typename T> __device__ __forceinline__ V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+// Warp's dot: (x*x' + y*y') + z*z'.
+template <typename T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> __device__ __forceinline__ T length(V3<T> a) { return msqrt(dot(a, a)); }
+template <typename T> __device__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ V3<T> xyz(V4<T> a) { return {a.x, a.y, a.z}; }
+template <typename T> __device__ __forceinline__ T sgn(T x) { return x < (T)0 ? (T)-1 : (T)1; }
+template <typename T> __device__ __forceinline__ T clampr(T x, T a, T b) {
+    T r = x < a ? a : x;
+    return r > b ? b : r;
+}
+
+// ---------------------------------------------------------------- physics
+#define SPHDEM_PI 3.14159265358979323846
+
+// kernel/Kernel_KNL.py:7-23, evaluated left to right as written there.
+template <typename T> __device__ __forceinline__ T wendland_w(T r, T h) {
+    T q = r / h;
+    T w = (T)0;
+    T u = (T)1.0 - (T)0.5 * q;
+    if (q < (T)2.0)
+        w = (T)21.0 * u * u * u * u * ((T)1.0 + (T)2.0 * q) / ((T)16.0 * (T)SPHDEM_PI * h * h * h);
+    return w;
+}
+// kernel/Kernel_KNL.py:26-41
+template <typename T> __device__ __forceinline__ T wendland_dw(T r, T h) {
+    T q = r / h;
+    T dw = (T)0;
+    if (q < (T)2.0) {
+        T u = (T)1.0 - (T)0.5 * q;
+        dw = (T)21.0 * ((T)-5.0 * q * u * u * u) / ((T)16.0 * (T)SPHDEM_PI * h * h * h * h);
+    }
+    return dw;
+}
+// kernel/KERNEL_pres.py:21-23
+template <typename T> __device__ __forceinline__ T tait(T rho, T rho0, T c0, T gamma) {
+    const T B = rho0 * c0 * c0 / gamma;
+    T q = mpow(rho / rho0, gamma) - (T)1.0;
+    return B * (q > (T)0 ? q : (T)0);
+}
+
+// ---------------------------------------------------------------- views
+template <typename T> struct Prm {
+    T h, support, rho0, c0, gamma, mu, g, dt, eps_min, eps_max;
+    T radius_max, boundary_radius_max, h_max, h_uniform;
+};
+template <typename T> inline Prm<T> prm(const sphdem_params_t *p) {
+    return {(T)p->h, (T)p->support, (T)p->rho0, (T)p->c0, (T)p->gamma, (T)p->mu, (T)p->g,
+            (T)p->dt, (T)p->eps_min, (T)p->eps_max, (T)p->radius_max,
+            (T)p->boundary_radius_max, (T)p->h_max, (T)p->h_uniform};
+}
+
+template <typename T> struct SphV {
+    T *pos, *vel, *rho_raw, *rho, *pres, *acc, *m, *flt, *porosity, *pgf, *acc_dem;
+    int n;
+};
+template <typename T> struct BndV { T *pos, *vel, *rho_raw, *rho, *pres, *acc, *m; int n; };
+template <typename T> struct DemV {
+    T *pos, *vel, *acc, *force, *omega, *torque, *radius, *rho, *m, *volume, *inertia, *K,
+      *eta, *mu, *h, *flt_s, *porosity, *drag, *pressure_force;
+    int *dem_off_old, *dem_id_old; T *dem_tang_old;
+    int *dem_off_new, *dem_id_new; T *dem_tang_new;
+    int *bnd_off_old, *bnd_id_old; T *bnd_tang_old;
+    int *bnd_off_new, *bnd_id_new; T *bnd_tang_new;
+    int cap_dem_new, cap_bnd_new, n;
+};
+template <typename T> struct WallV {
+    T *pos, *vel, *omega, *radius, *rho, *m, *volume, *inertia, *K, *eta, *mu;
+    int n;
+};
+template <typename T> struct GridV {
+    int dx, dy, dz, n;
+    T inv;
+    const int *ids, *lo, *hi;
+    const V4<T> *s;
+};
+template <typename T> struct WorkV {
+    T *fl_rho, *fl_pres, *fl_V;
+    V4<T> *fl_vel, *fl_pgf;
+    T *wl_rho, *wl_pres;
+    V4<T> *wl_vel;
+    int *scan_tmp, *flags;
+};
+
+template <typename T> inline SphV<T> view(const sphdem_sph_t *a) {
+    return {(T *)a->pos, (T *)a->vel, (T *)a->rho_raw, (T *)a->rho, (T *)a->pres, (T *)a->acc,
+            (T *)a->m, (T *)a->flt, (T *)a->porosity, (T *)a->pgf, (T *)a->acc_dem, a->n};
+}
+template <typename T> inline BndV<T> view(const sphdem_bnd_t *a) {
+    return {(T *)a->pos, (T *)a->vel, (T *)a->rho_raw, (T *)a->rho, (T *)a->pres, (T *)a->acc,
+            (T *)a->m, a->n};
+}
+template <typename T> inline DemV<T> view(const sphdem_dem_t *a) {
+    return {(T *)a->pos, (T *)a->vel, (T *)a->acc, (T *)a->force, (T *)a->omega, (T *)a->torque,
+            (T *)a->radius, (T *)a->rho, (T *)a->m, (T *)a->volume, (T *)a->inertia, (T *)a->K,
+            (T *)a->eta, (T *)a->mu, (T *)a->h, (T *)a->flt_s, (T *)a->porosity, (T *)a->drag,
+            (T *)a->pressure_force,
+            a->contact_dem_offset_old, a->contact_dem_id_old, (T *)a->tang_dem_old,
+            a->contact_dem_offset_new, a->contact_dem_id_new, (T *)a->tang_dem_new,
+            a->contact_bnd_offset_old, a->contact_bnd_id_old, (T *)a->tang_bnd_old,
+            a->contact_bnd_offset_new, a->contact_bnd_id_new, (T *)a->tang_bnd_new,
+            a->cap_dem_new, a->cap_bnd_new, a->n};
+}
+template <typename T> inline WallV<T> view(const sphdem_dembnd_t *a) {
+    return {(T *)a->pos, (T *)a->vel, (T *)a->omega, (T *)a->radius, (T *)a->rho, (T *)a->m,
+            (T *)a->volume, (T *)a->inertia, (T *)a->K, (T *)a->eta, (T *)a->mu, a->n};
+}
+template <typename T> inline GridV<T> view(const sphdem_grid_t *g) {
+    // Warp stores 1/cell_width rounded to the grid's float type.
+    T inv = (T)1.0 / (T)g->cell_width;
+    return {g->dim_x, g->dim_y, g->dim_z, g->n, inv, g->point_ids, g->cell_lo, g->cell_hi,
+            (const V4<T> *)g->sorted};
+}
+template <typename T> inline WorkV<T> view(const sphdem_work_t *w) {
+    return {(T *)w->fl_rho, (T *)w->fl_pres, (T *)w->fl_V, (V4<T> *)w->fl_vel,
+            (V4<T> *)w->fl_pgf, (T *)w->wl_rho, (T *)w->wl_pres, (V4<T> *)w->wl_vel,
+            w->scan_tmp, w->flags};
+}
+
+// ---------------------------------------------------------------- queries
+constexpr int kOrigin = 1 << 20;
+
+__device__ __forceinline__ int fold(int c, int dim) {
+    c += kOrigin;
+    return (c < 0 ? 0 : c) % dim;
+}
+
+// Visit the buckets of a HashGrid query in Warp's order -- z outer, y, x
+// inner, cells [int((p-r)/w), int((p+r)/w)] capped at start+dim-1 -- as
+// contiguous ranges [lo, hi) of sorted slots.  Consecutive x buckets of one
+// (y, z) row are adjacent in sorted order, so a row is one range, or two
+// when it wraps around the folded grid.  fn(lo, hi) sees the points in the
+// same order as Warp's per-bucket iteration.
+template <typename T, typename Fn>
+__device__ __forceinline__ void query_runs(const GridV<T> &g, T px, T py, T pz, T r, Fn &&fn) {
+    const T inv = g.inv;
+    const int x0 = (int)((px - r) * inv), y0 = (int)((py - r) * inv), z0 = (int)((pz - r) * inv);
+    const int x1 = min((int)((px + r) * inv), x0 + g.dx - 1);
+    const int y1 = min((int)((py + r) * inv), y0 + g.dy - 1);
+    const int z1 = min((int)((pz + r) * inv), z0 + g.dz - 1);
+    if (g.n == 0) return;
+    const int dxy = g.dx * g.dy;
+    const int len = x1 - x0 + 1;
+    for (int z = z0; z <= z1; ++z) {
+        const int cz = fold(z, g.dz);
+        for (int y = y0; y <= y1; ++y) {
+            const int base = cz * dxy + fold(y, g.dy) * g.dx;
+            if (x0 + kOrigin >= 0) {
+                const int cx = (x0 + kOrigin) % g.dx;
+                const int first = min(len, g.dx - cx);
+                if (first > 0) fn(g.lo[base + cx], g.hi[base + cx + first - 1]);
+                if (len > first) fn(g.lo[base], g.hi[base + len - first - 1]);
+            } else {
+                for (int x = x0; x <= x1; ++x) {
+                    const int c = base + fold(x, g.dx);
+                    fn(g.lo[c], g.hi[c]);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- wave64 sums
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace sphdem

@@ -1,0 +1,735 @@
+// dem.hip -- DEM contacts with CSR tangential history, SPH<->DEM coupling,
+// the fused coupled step, and FP64 diagnostics.
+//
+// Reference kernels: kernel/KERNEL_DEM_force.py, kernel/KERNEL_DEM_BC.py,
+// kernel/KERNEL_DEM_step.py, kernel/KERNEL_SPHDEM_interaction.py;
+// orchestration source/Simulation.py:102-221.
+#include "common.hpp"
+
+namespace sphdem {
+
+constexpr int kThreads = 256;
+
+// Declared in sph.hip.
+template <typename T>
+void launch_gather(const sphdem_sph_t *P, const sphdem_bnd_t *B, const sphdem_grid_t *gs,
+                   const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st);
+template <typename T>
+void launch_step_sph(const sphdem_params_t *p, const sphdem_sph_t *P, hipStream_t st);
+template <typename T>
+int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+              const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step, bool coupled,
+              const sphdem_work_t *w, hipStream_t st);
+template <typename T>
+void launch_pgf_only(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+                     const sphdem_grid_t *gs, const sphdem_grid_t *gb, const sphdem_work_t *w,
+                     hipStream_t st);
+int check_sph_pair(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+                   const sphdem_grid_t *gs, const sphdem_grid_t *gb);
+int check_work(const sphdem_work_t *w, int nf, int nb);
+
+// ---------------------------------------------------------------- contact law
+// kernel/KERNEL_DEM_force.py:25-74 DEM_contact: linear spring-dashpot with
+// unprojected tangential history and a componentwise Coulomb limit.
+template <typename T>
+__device__ __forceinline__ void dem_contact(V3<T> normal, T overlap, T ra, T rb, V3<T> va,
+                                            V3<T> vb, V3<T> wa, V3<T> wb, V3<T> tangent, T K,
+                                            T eta, T mu, T dt, V3<T> &f, V3<T> &disp,
+                                            V3<T> &tq) {
+    const V3<T> rc_a = (ra - (T)0.5 * overlap) * normal;
+    const V3<T> rc_b = (-(rb - (T)0.5 * overlap)) * normal;
+    const V3<T> urel = ((vb + cross(wb, rc_b)) - va) - cross(wa, rc_a);
+    const V3<T> un = dot(urel, normal) * normal;
+    const V3<T> us = urel - un;
+    disp = tangent + us * dt;
+    const V3<T> dir = us / (length(us) + (T)1.0e-20);
+    const V3<T> fn = (-K * overlap) * normal + eta * un;
+    const V3<T> fss = K * disp + eta * us;
+    const V3<T> fsf = (mu * K * overlap) * dir;
+    V3<T> fs = fss;
+    if (length(fss) >= length(fsf))
+        fs = mk3(sgn(fss.x) * mabs(fsf.x), sgn(fss.y) * mabs(fsf.y), sgn(fss.z) * mabs(fsf.z));
+    f = fn + fs;
+    tq = cross(rc_a, fs);
+}
+
+// KERNEL_DEM_force.py:9-22 and KERNEL_DEM_BC.py:10-24: overlap counts per row.
+// WALL selects the fixed-sphere set (no self exclusion).
+template <typename T, bool WALL>
+__global__ void k_count_contacts(DemV<T> D, GridV<T> g, const T *nb_radius, T rmax, int *off) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a == 0) off[D.n] = 0;
+    if (a >= D.n) return;
+    const V3<T> ra = ld3(D.pos, a);
+    const T Ra = D.radius[a];
+    int count = 0;
+    query_runs(g, ra.x, ra.y, ra.z, Ra + rmax, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const int b = g.ids[s];
+            if (!WALL && b == a) continue;
+            const V4<T> q = g.s[s];
+            const T dist = length(mk3(q.x - ra.x, q.y - ra.y, q.z - ra.z));
+            if (Ra + nb_radius[b] - dist > (T)0) ++count;
+        }
+    });
+    off[a] = count;
+}
+
+// KERNEL_DEM_force.py:77-135 (WALL = false) and KERNEL_DEM_BC.py:27-84
+// (WALL = true): loads plus the new CSR row; history looked up by stable id
+// in the old row (last match wins).  Rows past `cap` set flags bit 0.
+template <typename T, bool WALL>
+__global__ void k_contact_force(DemV<T> D, GridV<T> g, const T *nb_pos_unused, const T *nb_radius,
+                                const T *nb_vel, const T *nb_omega, T rmax, T grav, T dt,
+                                int *flags) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.n) return;
+    const int *off_old = WALL ? D.bnd_off_old : D.dem_off_old;
+    const int *id_old = WALL ? D.bnd_id_old : D.dem_id_old;
+    const T *tang_old = WALL ? D.bnd_tang_old : D.dem_tang_old;
+    const int *off_new = WALL ? D.bnd_off_new : D.dem_off_new;
+    int *id_new = WALL ? D.bnd_id_new : D.dem_id_new;
+    T *tang_new = WALL ? D.bnd_tang_new : D.dem_tang_new;
+    const int cap = WALL ? D.cap_bnd_new : D.cap_dem_new;
+    const V3<T> ra = ld3(D.pos, a);
+    const T Ra = D.radius[a];
+    const V3<T> va = ld3(D.vel, a), wa = ld3(D.omega, a);
+    const T K = D.K[a], eta = D.eta[a], mu = D.mu[a];
+    V3<T> force = WALL ? mk3<T>(0, 0, 0) : mk3<T>(0, 0, -D.m[a] * grav);
+    V3<T> torque = mk3<T>(0, 0, 0);
+    int out = off_new[a];
+    const int o0 = off_old[a], o1 = off_old[a + 1];
+    query_runs(g, ra.x, ra.y, ra.z, Ra + rmax, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const int b = g.ids[s];
+            if (!WALL && b == a) continue;
+            const V4<T> q = g.s[s];
+            const V3<T> rab = mk3(q.x - ra.x, q.y - ra.y, q.z - ra.z);
+            const T dist = length(rab);
+            const T overlap = Ra + nb_radius[b] - dist;
+            if (overlap > (T)0) {
+                V3<T> tangent = mk3<T>(0, 0, 0);
+                for (int k = o0; k < o1; ++k)
+                    if (id_old[k] == b) tangent = ld3(tang_old, k);
+                V3<T> normal = WALL ? mk3<T>(0, 0, -1) : mk3<T>(1, 0, 0);
+                if (!WALL && a > b) normal = -normal;
+                if (dist > (T)1.0e-12) normal = rab / dist;
+                V3<T> f, disp, tq;
+                dem_contact(normal, overlap, Ra, nb_radius[b], va, ld3(nb_vel, b), wa,
+                            ld3(nb_omega, b), tangent, K, eta, mu, dt, f, disp, tq);
+                force = force + f;
+                torque = torque + tq;
+                if (out < cap) {
+                    id_new[out] = b;
+                    st3(tang_new, out, disp);
+                } else {
+                    atomicOr(flags, 1);
+                }
+                ++out;
+            }
+        }
+    });
+    if (WALL) {
+        st3(D.force, a, ld3(D.force, a) + force);
+        st3(D.torque, a, ld3(D.torque, a) + torque);
+    } else {
+        st3(D.force, a, force);
+        st3(D.torque, a, torque);
+    }
+}
+
+// KERNEL_DEM_step.py:6-30
+template <typename T>
+__global__ void k_step_dem(DemV<T> D, T dt) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.n) return;
+    const V3<T> acc = ld3(D.force, a) / D.m[a];
+    const V3<T> vel = ld3(D.vel, a) + acc * dt;
+    st3(D.acc, a, acc);
+    st3(D.vel, a, vel);
+    st3(D.pos, a, ld3(D.pos, a) + vel * dt);
+    st3(D.omega, a, ld3(D.omega, a) + (ld3(D.torque, a) / D.inertia[a]) * dt);
+}
+
+// Copy a finished CSR (offsets n+1, E = off[n] edges) into the read side:
+// the no-round-trip step keeps buffer addresses fixed instead of swapping.
+template <typename T>
+__global__ void k_copy_csr(const int *off_s, const int *id_s, const T *tang_s, int *off_d,
+                           int *id_d, T *tang_d, int n, int cap) {
+    const int E = off_s[n];
+    const int lim = E < cap ? E : cap;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < (n + 1 > cap ? n + 1 : cap);
+         k += gridDim.x * blockDim.x) {
+        if (k <= n) off_d[k] = off_s[k];
+        if (k < lim) {
+            id_d[k] = id_s[k];
+            tang_d[3 * k] = tang_s[3 * k];
+            tang_d[3 * k + 1] = tang_s[3 * k + 1];
+            tang_d[3 * k + 2] = tang_s[3 * k + 2];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- coupling
+// kernel/KERNEL_SPHDEM_interaction.py:11-40
+template <typename T>
+__device__ __forceinline__ T drag_beta(T eps, T rho, T mu, T d, T speed) {
+    T beta = (T)0;
+    const T re = rho * d * speed / (mu + (T)1.0e-20);
+    if (eps <= (T)0.8) {
+        beta = (T)150.0 * ((T)1.0 - eps) / eps * mu / (d * d) + (T)1.75 * rho * speed / d;
+    } else {
+        T cd_speed = (T)0.44 * speed;
+        if (re <= (T)1000.0)
+            cd_speed = (T)24.0 * mu / (rho * d) * ((T)1.0 + (T)0.15 * mpow(re, (T)0.687));
+        beta = (T)0.75 * cd_speed * rho / d * mpow(eps, (T)-1.65);
+    }
+    return beta;
+}
+
+template <typename T>
+__global__ void k_drag_beta(const T *eps, const T *rho, const T *mu, const T *d, const T *speed,
+                            int n, T *beta) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) beta[i] = drag_beta(eps[i], rho[i], mu[i], d[i], speed[i]);
+}
+
+// Porosity half of Kernel_prep_sphdem (KERNEL_SPHDEM_interaction.py:74-96,
+// :111).  A subject with no DEM within 2*h_b has solid = 0, so its porosity
+// is clamp(1) whatever its coupling-scale fluid sum is; that sum is a local
+// the reference never stores, and it is only evaluated where a DEM is near.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_porosity(Prm<T> p, SphV<T> P, DemV<T> D,
+                                                       GridV<T> gs, GridV<T> gd, WorkV<T> w,
+                                                       T csup, T hu) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const V4<T> me = gs.s[k];
+    const int i = gs.ids[k];
+    bool near = false;
+    query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const V4<T> q = gd.s[s];
+            const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+            if (dist < (T)2.0 * D.h[gd.ids[s]]) near = true;
+        }
+    });
+    T solid = (T)0;
+    if (near) {
+        T flt = (T)0;
+        if (hu > (T)0) {
+            query_runs(gs, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+                for (int s = lo; s < hi; ++s) {
+                    const V4<T> q = gs.s[s];
+                    const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+                    if (dist < csup) flt = flt + w.fl_V[s] * wendland_w(dist, hu);
+                }
+            });
+        }
+        query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+            for (int s = lo; s < hi; ++s) {
+                const int b = gd.ids[s];
+                const V4<T> q = gd.s[s];
+                const T hb = D.h[b];
+                const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+                if (dist < (T)2.0 * hb) {
+                    T lflt = flt;
+                    if (hu <= (T)0) {
+                        lflt = (T)0;
+                        const T r2 = (T)2.0 * hb;
+                        query_runs(gs, me.x, me.y, me.z, r2, [&](int lo2, int hi2) {
+                            for (int s2 = lo2; s2 < hi2; ++s2) {
+                                const V4<T> q2 = gs.s[s2];
+                                const T fd = length(mk3(me.x - q2.x, me.y - q2.y, me.z - q2.z));
+                                if (fd < r2) lflt = lflt + w.fl_V[s2] * wendland_w(fd, hb);
+                            }
+                        });
+                    }
+                    solid = solid + D.volume[b] * wendland_w(dist, hb) / (lflt + (T)1.0e-20);
+                }
+            }
+        });
+    }
+    P.porosity[i] = clampr((T)1.0 - solid, p.eps_min, p.eps_max);
+}
+
+// kernel/KERNEL_SPHDEM_interaction.py:118-187, one workgroup per DEM
+// subject: the fluid sums are split over the workgroup's threads and added
+// in a fixed order (wave shuffles, then the four waves in turn).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_interaction_dem(Prm<T> p, DemV<T> D, GridV<T> gs,
+                                                              GridV<T> gd, WorkV<T> w) {
+    const int t = blockIdx.x;
+    if (t >= gd.n) return;
+    const int a = gd.ids[t];
+    const V3<T> ra = ld3(D.pos, a);
+    const T h = D.h[a], sup = (T)2.0 * h;
+    T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // flt, rho_f, vel_f xyz, pgf xyz
+    query_runs(gs, ra.x, ra.y, ra.z, sup, [&](int lo, int hi) {
+        for (int s = lo + (int)threadIdx.x; s < hi; s += blockDim.x) {
+            const V4<T> q = gs.s[s];
+            const T dist = length(mk3(ra.x - q.x, ra.y - q.y, ra.z - q.z));
+            if (dist < sup) {
+                const T vw = w.fl_V[s] * wendland_w(dist, h);
+                const V4<T> pr = w.fl_pgf[s];
+                const V4<T> v = w.fl_vel[s];
+                acc[0] = acc[0] + vw;
+                acc[1] = acc[1] + pr.w * vw;
+                acc[2] = acc[2] + v.x * vw;
+                acc[3] = acc[3] + v.y * vw;
+                acc[4] = acc[4] + v.z * vw;
+                acc[5] = acc[5] + pr.x * vw;
+                acc[6] = acc[6] + pr.y * vw;
+                acc[7] = acc[7] + pr.z * vw;
+            }
+        }
+    });
+    __shared__ T red[kThreads / 64][8];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const T v = wave_sum(acc[c]);
+        if (lane == 0) red[wid][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    T tot[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        T s = red[0][c];
+        for (int q = 1; q < kThreads / 64; ++q) s = s + red[q][c];
+        tot[c] = s;
+    }
+    const T flt = tot[0], rho_f = tot[1];
+    const V3<T> vel_f = mk3(tot[2], tot[3], tot[4]), pgf = mk3(tot[5], tot[6], tot[7]);
+    T solid = (T)0;
+    query_runs(gd, ra.x, ra.y, ra.z, sup, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const V4<T> q = gd.s[s];
+            const T dist = length(mk3(ra.x - q.x, ra.y - q.y, ra.z - q.z));
+            if (dist < sup) solid = solid + D.volume[gd.ids[s]] * wendland_w(dist, h);
+        }
+    });
+    D.flt_s[a] = flt;
+    st3(D.drag, a, mk3<T>(0, 0, 0));
+    st3(D.pressure_force, a, mk3<T>(0, 0, 0));
+    D.porosity[a] = p.eps_max;
+    if (flt > (T)1.0e-12) {
+        const T eps = clampr((T)1.0 - solid / flt, p.eps_min, p.eps_max);
+        const V3<T> urel = vel_f / flt - ld3(D.vel, a);
+        const T beta = drag_beta(eps, rho_f / flt, p.mu, (T)2.0 * D.radius[a], length(urel));
+        const T coef = beta * D.volume[a] / D.m[a];
+        const V3<T> drag = (D.m[a] * coef / ((T)1.0 + p.dt * coef)) * urel;
+        const V3<T> pf = ((D.m[a] / D.rho[a]) * pgf) / flt;
+        D.porosity[a] = eps;
+        st3(D.drag, a, drag);
+        st3(D.pressure_force, a, pf);
+        st3(D.force, a, (ld3(D.force, a) + drag) + pf);
+    }
+}
+
+// kernel/KERNEL_SPHDEM_interaction.py:190-226 (INTER) fused with
+// kernel/KERNEL_step.py:6-22 (STEP): both only touch the subject's own fields.
+template <typename T, bool INTER, bool STEP>
+__global__ __launch_bounds__(kThreads) void k_fluid_final(Prm<T> p, SphV<T> P, DemV<T> D,
+                                                          GridV<T> gs, GridV<T> gd, T csup) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const int i = gs.ids[k];
+    V3<T> acc = ld3(P.acc, i);
+    if (INTER) {
+        const V4<T> me = gs.s[k];
+        V3<T> fd = mk3<T>(0, 0, 0);
+        query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+            for (int s = lo; s < hi; ++s) {
+                const int b = gd.ids[s];
+                const V4<T> q = gd.s[s];
+                const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+                const T hb = D.h[b];
+                if (dist < (T)2.0 * hb && D.flt_s[b] > (T)1.0e-12)
+                    fd = fd - (ld3(D.drag, b) * wendland_w(dist, hb)) / D.flt_s[b];
+            }
+        });
+        const V3<T> ad = fd / (P.rho[i] * P.porosity[i]);
+        st3(P.acc_dem, i, ad);
+        acc = acc + ad;
+        st3(P.acc, i, acc);
+    }
+    if (STEP) {
+        const V3<T> v = ld3(P.vel, i) + acc * p.dt;
+        st3(P.vel, i, v);
+        st3(P.pos, i, ld3(P.pos, i) + v * p.dt);
+    }
+}
+
+// ---------------------------------------------------------------- diagnostics
+template <typename T>
+__global__ __launch_bounds__(1024) void k_reduce(const T *pos, const T *vel, const T *m, int nf,
+                                                 const T *dvel, const T *dw, const T *dm,
+                                                 const T *dI, int nd, double *out) {
+    double s_m = 0, s_kf = 0, s_kd = 0, s_md = 0, front = -1.0e300;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        const double mi = (double)m[i];
+        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        s_m += mi;
+        s_kf += 0.5 * mi * (vx * vx + vy * vy + vz * vz);
+        const double x = (double)pos[3 * i];
+        front = x > front ? x : front;
+    }
+    for (int a = threadIdx.x; a < nd; a += blockDim.x) {
+        const double ma = (double)dm[a], Ia = (double)dI[a];
+        const double vx = dvel[3 * a], vy = dvel[3 * a + 1], vz = dvel[3 * a + 2];
+        const double wx = dw[3 * a], wy = dw[3 * a + 1], wz = dw[3 * a + 2];
+        s_md += ma;
+        s_kd += 0.5 * ma * (vx * vx + vy * vy + vz * vz) + 0.5 * Ia * (wx * wx + wy * wy + wz * wz);
+    }
+    __shared__ double red[16][5];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double v[4] = {s_m, s_kf, s_kd, s_md};
+    for (int c = 0; c < 4; ++c) {
+        const double r = wave_sum(v[c]);
+        if (lane == 0) red[wid][c] = r;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(front, off, 64);
+        front = o > front ? o : front;
+    }
+    if (lane == 0) red[wid][4] = front;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[5] = {0, 0, 0, 0, -1.0e300};
+        for (int q = 0; q < (int)(blockDim.x / 64); ++q) {
+            for (int c = 0; c < 4; ++c) t[c] += red[q][c];
+            t[4] = red[q][4] > t[4] ? red[q][4] : t[4];
+        }
+        out[0] = t[0];
+        out[1] = t[1];
+        out[2] = t[2];
+        out[3] = t[3];
+        out[4] = t[4];
+    }
+}
+
+// ---------------------------------------------------------------- host side
+static int check_dem(const sphdem_params_t *p, const sphdem_dem_t *D, const sphdem_grid_t *gd) {
+    SPHDEM_REQUIRE(p && D && gd, "NULL argument");
+    SPHDEM_REQUIRE(p->precision == 4 || p->precision == 8, "precision must be 4 or 8");
+    SPHDEM_REQUIRE(gd->precision == p->precision, "grid precision differs from the DEM precision");
+    SPHDEM_REQUIRE(D->n >= 0, "DEM count must be nonnegative");
+    SPHDEM_REQUIRE(D->n == 0 || (D->pos && D->vel && D->force && D->omega && D->torque &&
+                                 D->radius && D->m && D->inertia && D->K && D->eta && D->mu &&
+                                 D->h && D->volume && D->rho),
+                   "DEM field pointer is NULL");
+    SPHDEM_REQUIRE(gd->n == 0 || (gd->point_ids && gd->sorted), "DEM grid is not built");
+    return SPHDEM_OK;
+}
+
+static int check_csr(const sphdem_dem_t *D, bool wall) {
+    const int *oo = wall ? D->contact_bnd_offset_old : D->contact_dem_offset_old;
+    const int *on = wall ? D->contact_bnd_offset_new : D->contact_dem_offset_new;
+    SPHDEM_REQUIRE(oo && on, "CSR offset buffer is NULL");
+    const int cap = wall ? D->cap_bnd_new : D->cap_dem_new;
+    SPHDEM_REQUIRE(cap >= 0, "CSR capacity must be nonnegative");
+    SPHDEM_REQUIRE(cap == 0 || (wall ? (D->contact_bnd_id_new && D->tang_bnd_new)
+                                     : (D->contact_dem_id_new && D->tang_dem_new)),
+                   "CSR edge buffer is NULL");
+    return SPHDEM_OK;
+}
+
+template <typename T, bool WALL>
+static void launch_count(const sphdem_dem_t *D, const sphdem_grid_t *g, const void *nb_radius,
+                         double rmax, hipStream_t st) {
+    int *off = WALL ? D->contact_bnd_offset_new : D->contact_dem_offset_new;
+    hipLaunchKernelGGL((k_count_contacts<T, WALL>), dim3(blocks_for(D->n + 1, 64)), dim3(64), 0, st,
+                       view<T>(D), view<T>(g), (const T *)nb_radius, (T)rmax, off);
+}
+
+template <typename T, bool WALL>
+static void launch_contact(const sphdem_dem_t *D, const sphdem_grid_t *g, const void *nb_radius,
+                           const void *nb_vel, const void *nb_omega, double rmax, double grav,
+                           double dt, int *flags, hipStream_t st) {
+    if (D->n == 0) return;
+    hipLaunchKernelGGL((k_contact_force<T, WALL>), dim3(blocks_for(D->n, 64)), dim3(64), 0, st,
+                       view<T>(D), view<T>(g), (const T *)nullptr, (const T *)nb_radius,
+                       (const T *)nb_vel, (const T *)nb_omega, (T)rmax, (T)grav, (T)dt, flags);
+}
+
+template <typename T>
+static void launch_copy_csr(const sphdem_dem_t *D, bool wall, hipStream_t st) {
+    const int cap = wall ? D->cap_bnd_new : D->cap_dem_new;
+    const int span = (D->n + 1 > cap ? D->n + 1 : cap);
+    if (wall)
+        hipLaunchKernelGGL(k_copy_csr<T>, dim3(blocks_for(span, 256)), dim3(256), 0, st,
+                           D->contact_bnd_offset_new, D->contact_bnd_id_new,
+                           (const T *)D->tang_bnd_new, D->contact_bnd_offset_old,
+                           D->contact_bnd_id_old, (T *)D->tang_bnd_old, D->n, cap);
+    else
+        hipLaunchKernelGGL(k_copy_csr<T>, dim3(blocks_for(span, 256)), dim3(256), 0, st,
+                           D->contact_dem_offset_new, D->contact_dem_id_new,
+                           (const T *)D->tang_dem_new, D->contact_dem_offset_old,
+                           D->contact_dem_id_old, (T *)D->tang_dem_old, D->n, cap);
+}
+
+template <typename T>
+static void launch_porosity(const sphdem_params_t *p, const sphdem_sph_t *P,
+                            const sphdem_dem_t *D, const sphdem_grid_t *gs,
+                            const sphdem_grid_t *gd, const sphdem_work_t *w, double csup,
+                            double hu, hipStream_t st) {
+    if (gs->n == 0) return;
+    hipLaunchKernelGGL(k_porosity<T>, dim3(blocks_for(gs->n, kThreads)), dim3(kThreads), 0, st,
+                       prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), view<T>(w),
+                       (T)csup, (T)hu);
+}
+
+template <typename T>
+static void launch_interaction_dem(const sphdem_params_t *p, const sphdem_dem_t *D,
+                                   const sphdem_grid_t *gs, const sphdem_grid_t *gd,
+                                   const sphdem_work_t *w, hipStream_t st) {
+    if (gd->n == 0) return;
+    hipLaunchKernelGGL(k_interaction_dem<T>, dim3(gd->n), dim3(kThreads), 0, st, prm<T>(p),
+                       view<T>(D), view<T>(gs), view<T>(gd), view<T>(w));
+}
+
+template <typename T, bool INTER, bool STEP>
+static void launch_fluid_final(const sphdem_params_t *p, const sphdem_sph_t *P,
+                               const sphdem_dem_t *D, const sphdem_grid_t *gs,
+                               const sphdem_grid_t *gd, double csup, hipStream_t st) {
+    if (gs->n == 0) return;
+    hipLaunchKernelGGL((k_fluid_final<T, INTER, STEP>), dim3(blocks_for(gs->n, kThreads)),
+                       dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs),
+                       view<T>(gd), (T)csup);
+}
+
+template <typename T>
+static void launch_step_dem(const sphdem_params_t *p, const sphdem_dem_t *D, hipStream_t st) {
+    if (D->n == 0) return;
+    hipLaunchKernelGGL(k_step_dem<T>, dim3(blocks_for(D->n, 64)), dim3(64), 0, st, view<T>(D),
+                       (T)p->dt);
+}
+
+// Simulation.py:102-221; see sphdem_coupled_step in sphdem.h for the stages.
+template <typename T>
+static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+                        const sphdem_dem_t *D, const sphdem_dembnd_t *W, const sphdem_grid_t *gs,
+                        const sphdem_grid_t *gb, const sphdem_grid_t *gd,
+                        const sphdem_grid_t *gw, int step, int stage, const sphdem_work_t *w,
+                        hipStream_t st) {
+    const bool all = stage < 0;
+    int rc;
+    if (all || stage == 0) {
+        rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
+        if (rc) return rc;
+        launch_count<T, false>(D, gd, D->radius, p->radius_max, st);
+        rc = exclusive_scan(D->contact_dem_offset_new, D->contact_dem_offset_new, D->n + 1,
+                            w->scan_tmp, st);
+        if (rc) return rc;
+    }
+    if (all || stage == 1) {
+        launch_contact<T, false>(D, gd, D->radius, D->vel, D->omega, p->radius_max, p->g, p->dt,
+                                 w->flags, st);
+        if (all) launch_copy_csr<T>(D, false, st);
+        launch_count<T, true>(D, gw, W->radius, p->boundary_radius_max, st);
+        rc = exclusive_scan(D->contact_bnd_offset_new, D->contact_bnd_offset_new, D->n + 1,
+                            w->scan_tmp, st);
+        if (rc) return rc;
+    }
+    if (all || stage == 2) {
+        launch_contact<T, true>(D, gw, W->radius, W->vel, W->omega, p->boundary_radius_max, 0.0,
+                                p->dt, w->flags, st);
+        if (all) launch_copy_csr<T>(D, true, st);
+        const double csup = 2.0 * p->h_max;
+        launch_porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st);
+        launch_interaction_dem<T>(p, D, gs, gd, w, st);
+        launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st);
+        launch_step_dem<T>(p, D, st);
+    }
+    return check_launch("coupled_step");
+}
+
+}  // namespace sphdem
+
+using namespace sphdem;
+
+#define DISPATCH(prec, CALL)                 \
+    do {                                     \
+        if ((prec) == 4) {                   \
+            using T = float;                 \
+            CALL;                            \
+        } else {                             \
+            using T = double;                \
+            CALL;                            \
+        }                                    \
+    } while (0)
+
+extern "C" {
+
+int sphdem_drag_beta(int precision, const void *eps, const void *rho, const void *mu,
+                     const void *diameter, const void *speed, int n, void *beta, void *stream) {
+    SPHDEM_REQUIRE(precision == 4 || precision == 8, "precision must be 4 or 8");
+    SPHDEM_REQUIRE(n >= 0 && (n == 0 || (eps && rho && mu && diameter && speed && beta)),
+                   "invalid drag buffers");
+    if (n == 0) return SPHDEM_OK;
+    DISPATCH(precision, hipLaunchKernelGGL(k_drag_beta<T>, dim3(blocks_for(n, 256)), dim3(256), 0,
+                                           as_stream(stream), (const T *)eps, (const T *)rho,
+                                           (const T *)mu, (const T *)diameter, (const T *)speed, n,
+                                           (T *)beta));
+    return check_launch("drag_beta");
+}
+
+int sphdem_count_dem_contacts(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_grid_t *gd,
+                              double radius_max, void *stream) {
+    int rc = check_dem(p, D, gd);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(D->contact_dem_offset_new, "contact_dem_offset_new is NULL");
+    SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+    DISPATCH(p->precision, (launch_count<T, false>(D, gd, D->radius, radius_max, as_stream(stream))));
+    return check_launch("count_dem_contacts");
+}
+
+int sphdem_force_dem(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_grid_t *gd,
+                     double radius_max, double g, int *flags, void *stream) {
+    int rc = check_dem(p, D, gd);
+    if (rc) return rc;
+    rc = check_csr(D, false);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(flags, "flags buffer is NULL");
+    SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+    DISPATCH(p->precision, (launch_contact<T, false>(D, gd, D->radius, D->vel, D->omega, radius_max,
+                                                     g, p->dt, flags, as_stream(stream))));
+    return check_launch("force_dem");
+}
+
+int sphdem_count_bnd_contacts(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_dembnd_t *W,
+                              const sphdem_grid_t *gw, double radius_max, void *stream) {
+    int rc = check_dem(p, D, gw);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(W && D->contact_bnd_offset_new, "NULL argument");
+    SPHDEM_REQUIRE(gw->n == W->n, "DEM-wall grid was built from a different number of points");
+    DISPATCH(p->precision, (launch_count<T, true>(D, gw, W->radius, radius_max, as_stream(stream))));
+    return check_launch("count_bnd_contacts");
+}
+
+int sphdem_bc_dem(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_dembnd_t *W,
+                  const sphdem_grid_t *gw, double radius_max, int *flags, void *stream) {
+    int rc = check_dem(p, D, gw);
+    if (rc) return rc;
+    rc = check_csr(D, true);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(W && flags, "NULL argument");
+    SPHDEM_REQUIRE(gw->n == W->n, "DEM-wall grid was built from a different number of points");
+    DISPATCH(p->precision, (launch_contact<T, true>(D, gw, W->radius, W->vel, W->omega, radius_max,
+                                                    0.0, p->dt, flags, as_stream(stream))));
+    return check_launch("bc_dem");
+}
+
+int sphdem_step_dem(const sphdem_params_t *p, sphdem_dem_t *D, void *stream) {
+    SPHDEM_REQUIRE(p && D && (p->precision == 4 || p->precision == 8), "invalid arguments");
+    SPHDEM_REQUIRE(D->n == 0 || (D->pos && D->vel && D->acc && D->force && D->omega && D->torque &&
+                                 D->m && D->inertia),
+                   "DEM field pointer is NULL");
+    DISPATCH(p->precision, launch_step_dem<T>(p, D, as_stream(stream)));
+    return check_launch("step_dem");
+}
+
+int sphdem_prep_sphdem(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       sphdem_dem_t *D, const sphdem_grid_t *gs, const sphdem_grid_t *gb,
+                       const sphdem_grid_t *gd, double support, double h,
+                       const sphdem_work_t *w, void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    rc = check_dem(p, D, gd);
+    if (rc) return rc;
+    rc = check_work(w, P->n, B->n);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(P->n == 0 || (P->porosity && P->pgf), "fluid coupling field is NULL");
+    SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+    hipStream_t st = as_stream(stream);
+    DISPATCH(p->precision, (launch_pgf_only<T>(p, P, B, gs, gb, w, st),
+                            launch_porosity<T>(p, P, D, gs, gd, w, support, h, st)));
+    return check_launch("prep_sphdem");
+}
+
+int sphdem_interaction_dem(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_dem_t *D,
+                           const sphdem_grid_t *gs, const sphdem_grid_t *gd,
+                           const sphdem_work_t *w, void *stream) {
+    int rc = check_dem(p, D, gd);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(P && gs && gs->n == P->n, "fluid grid was built from a different number of points");
+    SPHDEM_REQUIRE(gs->precision == p->precision, "grid precision differs");
+    rc = check_work(w, P->n, 0);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+    SPHDEM_REQUIRE(D->n == 0 || (D->flt_s && D->porosity && D->drag && D->pressure_force),
+                   "DEM coupling field is NULL");
+    hipStream_t st = as_stream(stream);
+    DISPATCH(p->precision, (launch_gather<T>(P, nullptr, gs, nullptr, w, st),
+                            launch_interaction_dem<T>(p, D, gs, gd, w, st)));
+    return check_launch("interaction_dem");
+}
+
+int sphdem_interaction_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_dem_t *D,
+                           const sphdem_grid_t *gs, const sphdem_grid_t *gd, double support,
+                           void *stream) {
+    int rc = check_dem(p, D, gd);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(P && gs && gs->n == P->n, "fluid grid was built from a different number of points");
+    SPHDEM_REQUIRE(gs->precision == p->precision, "grid precision differs");
+    SPHDEM_REQUIRE(P->n == 0 || (P->acc && P->acc_dem && P->rho && P->porosity),
+                   "fluid field pointer is NULL");
+    SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+    DISPATCH(p->precision,
+             (launch_fluid_final<T, true, false>(p, P, D, gs, gd, support, as_stream(stream))));
+    return check_launch("interaction_sph");
+}
+
+int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                        sphdem_dem_t *D, sphdem_dembnd_t *W, const sphdem_grid_t *gs,
+                        const sphdem_grid_t *gb, const sphdem_grid_t *gd,
+                        const sphdem_grid_t *gw, int step, int stage, const sphdem_work_t *w,
+                        void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    rc = check_dem(p, D, gd);
+    if (rc) return rc;
+    rc = check_dem(p, D, gw);
+    if (rc) return rc;
+    rc = check_work(w, P->n, B->n);
+    if (rc) return rc;
+    rc = check_csr(D, false);
+    if (rc) return rc;
+    rc = check_csr(D, true);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(W != nullptr, "DEM wall is NULL");
+    SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+    SPHDEM_REQUIRE(gw->n == W->n, "DEM-wall grid was built from a different number of points");
+    SPHDEM_REQUIRE(D->n > 0, "coupled step needs DEM particles");
+    SPHDEM_REQUIRE(stage >= -1 && stage <= 2, "stage must be -1, 0, 1 or 2");
+    SPHDEM_REQUIRE(step >= 0, "step must be nonnegative");
+    SPHDEM_REQUIRE(w->scan_tmp && w->flags, "work scan/flags buffer is NULL");
+    SPHDEM_REQUIRE(P->n == 0 || (P->porosity && P->pgf && P->acc_dem), "fluid coupling field is NULL");
+    if (stage < 0) {
+        SPHDEM_REQUIRE(D->contact_dem_id_old && D->tang_dem_old && D->contact_bnd_id_old &&
+                           D->tang_bnd_old,
+                       "old CSR edge buffer is NULL");
+    }
+    DISPATCH(p->precision,
+             rc = coupled_step<T>(p, P, B, D, W, gs, gb, gd, gw, step, stage, w, as_stream(stream)));
+    return rc;
+}
+
+int sphdem_reduce_f64(int precision, const void *sph_pos, const void *sph_vel, const void *sph_m,
+                      int nf, const void *dem_vel, const void *dem_omega, const void *dem_m,
+                      const void *dem_inertia, int nd, double *out, void *stream) {
+    SPHDEM_REQUIRE(precision == 4 || precision == 8, "precision must be 4 or 8");
+    SPHDEM_REQUIRE(out && nf >= 0 && nd >= 0, "invalid reduction arguments");
+    SPHDEM_REQUIRE(nf == 0 || (sph_pos && sph_vel && sph_m), "fluid buffer is NULL");
+    SPHDEM_REQUIRE(nd == 0 || (dem_vel && dem_omega && dem_m && dem_inertia), "DEM buffer is NULL");
+    DISPATCH(precision, hipLaunchKernelGGL(k_reduce<T>, dim3(1), dim3(1024), 0, as_stream(stream),
+                                           (const T *)sph_pos, (const T *)sph_vel,
+                                           (const T *)sph_m, nf, (const T *)dem_vel,
+                                           (const T *)dem_omega, (const T *)dem_m,
+                                           (const T *)dem_inertia, nd, out));
+    return check_launch("reduce_f64");
+}
+
+}  // extern "C"

@@ -1,0 +1,306 @@
+// grid.hip -- device-wide exclusive scan and the counting-sort hash grid.
+//
+// Replaces wp.HashGrid (build: main.py:66,73,85,87; queries in every
+// neighbour kernel) and wp.utils.array_scan (source/Simulation.py:148,181).
+//
+// Build of n points into dim_x*dim_y*dim_z buckets:
+//   1. key:     point_cells[i] = bucket(int(p*inv)); rank[i] = atomicAdd(count[key])
+//   2. scan:    cell_lo = exclusive_scan(count)
+//   3. scatter: point_ids[cell_lo[key] + rank[i]] = i
+//   4. fix:     cell_hi = cell_lo + count; each bucket's ids sorted ascending,
+//               which makes the order equal to a stable sort by key and
+//               independent of the order the atomics were served in
+//   5. sorted:  sorted[k] = (p[id], w[id]) for coalesced neighbour reads
+#include "common.hpp"
+
+#include <cstdio>
+
+namespace sphdem {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SPHDEM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return SPHDEM_OK;
+}
+
+// ---------------------------------------------------------------- scan
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Exclusive scan of one tile of `in` into `out`, offset by `carry`; returns
+// the tile total to every thread.
+__device__ int scan_tile(const int *in, int *out, int n, int tile_base, int carry) {
+    __shared__ int wsum[kScanThreads / 64];
+    __shared__ int total;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int base = tile_base + t * kScanItems;
+    int v[kScanItems];
+    int tsum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int idx = base + k;
+        const int x = idx < n ? in[idx] : 0;
+        v[k] = tsum;
+        tsum += x;
+    }
+    int incl = tsum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int wofs = 0;
+    for (int w = 0; w < wid; ++w) wofs += wsum[w];
+    const int excl = carry + wofs + incl - tsum;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int idx = base + k;
+        if (idx < n) out[idx] = v[k] + excl;
+    }
+    if (t == kScanThreads - 1) total = wofs + incl;
+    __syncthreads();
+    const int r = total;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const int *in, int *out, int n,
+                                                             int *partial) {
+    const int tot = scan_tile(in, out, n, blockIdx.x * kScanTile, 0);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// One block scans the tile totals in place (any count, chunk by chunk).
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(int *partial, int tiles) {
+    int carry = 0;
+    for (int base = 0; base < tiles; base += kScanTile)
+        carry += scan_tile(partial, partial, tiles, base, carry);
+}
+
+__global__ void k_scan_add(int *out, int n, const int *partial) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) out[idx] += partial[idx / kScanTile];
+}
+
+int scan_scratch(int n) { return blocks_for(n, kScanTile) + 1; }
+
+int exclusive_scan(const int *in, int *out, int n, int *scratch, hipStream_t st) {
+    if (n <= 0) return SPHDEM_OK;
+    const int tiles = blocks_for(n, kScanTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, st, in, out, n, scratch);
+    if (tiles > 1) {
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, st, scratch, tiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, st, out, n, scratch);
+    }
+    return check_launch("exclusive_scan");
+}
+
+// ---------------------------------------------------------------- grid build
+__global__ void k_fill_int(int *a, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+template <typename T>
+__global__ void k_grid_key(const T *pos, int n, T inv, int dx, int dy, int dz, int *cells,
+                           int *rank, int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)(pos[3 * i] * inv), y = (int)(pos[3 * i + 1] * inv),
+              z = (int)(pos[3 * i + 2] * inv);
+    const int key = fold(z, dz) * (dx * dy) + fold(y, dy) * dx + fold(x, dx);
+    cells[i] = key;
+    rank[i] = atomicAdd(&count[key], 1);
+}
+
+__global__ void k_grid_scatter(const int *cells, const int *rank, const int *lo, int n, int *ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ids[lo[cells[i]] + rank[i]] = i;
+}
+
+// cell_hi holds the counts on entry and the bucket ends on exit.
+__global__ void k_grid_fix(const int *lo, int *hi, int *ids, int ncell) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int a = lo[c], cnt = hi[c];
+    hi[c] = a + cnt;
+    for (int k = a + 1; k < a + cnt; ++k) {
+        const int v = ids[k];
+        int j = k - 1;
+        while (j >= a && ids[j] > v) {
+            ids[j + 1] = ids[j];
+            --j;
+        }
+        ids[j + 1] = v;
+    }
+}
+
+template <typename T>
+__global__ void k_grid_sorted(const T *pos, const T *w, const int *ids, int n, V4<T> *s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = ids[k];
+    s[k] = V4<T>{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], w ? w[i] : (T)0};
+}
+
+template <typename T>
+static int grid_build(sphdem_grid_t *g, const void *points, int n, double radius, const void *w,
+                      hipStream_t st) {
+    const int ncell = g->dim_x * g->dim_y * g->dim_z;
+    const T inv = (T)1.0 / (T)radius;
+    g->cell_width = radius;
+    g->n = n;
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, g->cell_hi, ncell, 0);
+    if (n > 0)
+        hipLaunchKernelGGL(k_grid_key<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const T *)points, n, inv, g->dim_x, g->dim_y, g->dim_z, g->point_cells,
+                           g->rank, g->cell_hi);
+    int rc = exclusive_scan(g->cell_hi, g->cell_lo, ncell, g->scan_tmp, st);
+    if (rc) return rc;
+    if (n > 0)
+        hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           g->point_cells, g->rank, g->cell_lo, n, g->point_ids);
+    hipLaunchKernelGGL(k_grid_fix, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, g->cell_lo,
+                       g->cell_hi, g->point_ids, ncell);
+    if (n > 0)
+        hipLaunchKernelGGL(k_grid_sorted<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const T *)points, (const T *)w, g->point_ids, n, (V4<T> *)g->sorted);
+    return check_launch("grid_build");
+}
+
+template <typename T>
+__global__ void k_grid_set_w(const T *w, const int *ids, int n, V4<T> *s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) s[k].w = w[ids[k]];
+}
+
+__global__ void k_warp_bounds(const int *lo, const int *hi, int ncell, int *starts, int *ends) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const bool any = hi[c] > lo[c];
+    starts[c] = any ? lo[c] : 0;
+    ends[c] = any ? hi[c] : 0;
+}
+
+template <typename T>
+__global__ void k_neighbour_counts(GridV<T> gs, GridV<T> gn, T r, int *counts) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const V4<T> me = gs.s[k];
+    const T rr = r * r;
+    int c = 0;
+    query_runs(gn, me.x, me.y, me.z, r, [&](int a, int b) {
+        for (int s = a; s < b; ++s) {
+            const V4<T> q = gn.s[s];
+            const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+            if (dx * dx + dy * dy + dz * dz < rr) ++c;
+        }
+    });
+    counts[gs.ids[k]] = c;
+}
+
+static int check_grid(const sphdem_grid_t *g) {
+    SPHDEM_REQUIRE(g != nullptr, "grid is NULL");
+    SPHDEM_REQUIRE(g->precision == 4 || g->precision == 8, "grid precision must be 4 or 8");
+    SPHDEM_REQUIRE(g->dim_x > 0 && g->dim_y > 0 && g->dim_z > 0, "grid dimensions must be positive");
+    SPHDEM_REQUIRE((long)g->dim_x * g->dim_y * g->dim_z < (1L << 30), "grid has too many buckets");
+    SPHDEM_REQUIRE(g->cell_lo && g->cell_hi && g->scan_tmp, "grid bucket buffers are NULL");
+    SPHDEM_REQUIRE(g->n == 0 || (g->point_ids && g->sorted), "grid point buffers are NULL");
+    return SPHDEM_OK;
+}
+
+}  // namespace sphdem
+
+using namespace sphdem;
+
+extern "C" {
+
+int sphdem_version(void) { return 1; }
+
+const char *sphdem_last_error(void) { return g_err.c_str(); }
+
+int sphdem_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int sphdem_scan_scratch(int n) { return scan_scratch(n); }
+
+int sphdem_exclusive_scan(int *a, int n, int *scratch, void *stream) {
+    SPHDEM_REQUIRE(n >= 0, "scan length must be nonnegative");
+    SPHDEM_REQUIRE(n == 0 || (a && scratch), "scan buffers are NULL");
+    return exclusive_scan(a, a, n, scratch, as_stream(stream));
+}
+
+int sphdem_grid_build(sphdem_grid_t *g, const void *points, int n, double radius, const void *w,
+                      void *stream) {
+    SPHDEM_REQUIRE(g != nullptr, "grid is NULL");
+    SPHDEM_REQUIRE(n >= 0, "point count must be nonnegative");
+    SPHDEM_REQUIRE(radius > 0.0, "grid radius must be positive");
+    SPHDEM_REQUIRE(n == 0 || (points && g->point_cells && g->point_ids && g->rank && g->sorted),
+                   "grid build buffers are NULL");
+    g->n = n;
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (g->precision == 4) return grid_build<float>(g, points, n, radius, w, as_stream(stream));
+    return grid_build<double>(g, points, n, radius, w, as_stream(stream));
+}
+
+int sphdem_grid_set_w(sphdem_grid_t *g, const void *w, void *stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(g->n == 0 || w, "w is NULL");
+    if (g->n == 0) return SPHDEM_OK;
+    if (g->precision == 4)
+        hipLaunchKernelGGL(k_grid_set_w<float>, dim3(blocks_for(g->n, 256)), dim3(256), 0,
+                           as_stream(stream), (const float *)w, g->point_ids, g->n,
+                           (V4<float> *)g->sorted);
+    else
+        hipLaunchKernelGGL(k_grid_set_w<double>, dim3(blocks_for(g->n, 256)), dim3(256), 0,
+                           as_stream(stream), (const double *)w, g->point_ids, g->n,
+                           (V4<double> *)g->sorted);
+    return check_launch("grid_set_w");
+}
+
+int sphdem_grid_warp_bounds(const sphdem_grid_t *g, int *cell_starts, int *cell_ends, void *stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(cell_starts && cell_ends, "bounds buffers are NULL");
+    const int ncell = g->dim_x * g->dim_y * g->dim_z;
+    hipLaunchKernelGGL(k_warp_bounds, dim3(blocks_for(ncell, 256)), dim3(256), 0, as_stream(stream),
+                       g->cell_lo, g->cell_hi, ncell, cell_starts, cell_ends);
+    return check_launch("grid_warp_bounds");
+}
+
+int sphdem_neighbour_counts(const sphdem_grid_t *gs, const sphdem_grid_t *gn, double radius,
+                            int *counts, void *stream) {
+    int rc = check_grid(gs);
+    if (rc) return rc;
+    rc = check_grid(gn);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(gs->precision == gn->precision, "grids differ in precision");
+    SPHDEM_REQUIRE(gs->n == 0 || counts, "counts buffer is NULL");
+    if (gs->n == 0) return SPHDEM_OK;
+    if (gs->precision == 4)
+        hipLaunchKernelGGL(k_neighbour_counts<float>, dim3(blocks_for(gs->n, 256)), dim3(256), 0,
+                           as_stream(stream), view<float>(gs), view<float>(gn), (float)radius, counts);
+    else
+        hipLaunchKernelGGL(k_neighbour_counts<double>, dim3(blocks_for(gs->n, 256)), dim3(256), 0,
+                           as_stream(stream), view<double>(gs), view<double>(gn), radius, counts);
+    return check_launch("neighbour_counts");
+}
+
+}  // extern "C"

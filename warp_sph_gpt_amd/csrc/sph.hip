@@ -1,0 +1,437 @@
+// sph.hip -- WCSPH neighbour passes on the hash grid.
+//
+// Reference kernels: kernel/KERNEL_rho.py (Shepard, fluid and wall density),
+// kernel/KERNEL_pres.py (Tait EOS), kernel/KERNEL_force.py (pressure +
+// Morris viscosity + gravity), kernel/KERNEL_step.py (semi-implicit Euler),
+// and the SPH-scale pressure-gradient half of Kernel_prep_sphdem
+// (kernel/KERNEL_SPHDEM_interaction.py:98-113).
+//
+// One thread per subject, subjects in grid-sorted order (the reference's
+// hash_grid_point_id(grid, tid)), neighbours read from the grid's sorted
+// copy (x, y, z, m) in Warp's bucket order, so every per-subject sum adds
+// the same terms in the same order as the reference's loop.  Per-step
+// working arrays (work.fl_*, work.wl_*) hold densities, pressures and
+// velocities in sorted order, so the force pass reads neighbours
+// contiguously instead of gathering by particle index.
+#include "common.hpp"
+
+namespace sphdem {
+
+constexpr int kThreads = 256;
+
+// Fluid subjects: Shepard filter and/or density (KERNEL_rho.py:6-82).
+//  SHEP: flt_i = sum (m_j/rho_j) W over fluid and wall, stored in P.flt.
+//  DENS: rho_i = sum m_j W / flt_i.
+//  WORK: the fused step -- new density, Tait pressure and velocity go to the
+//        sorted working arrays (P.rho is still read as the old density by
+//        the Shepard sum of other threads, so it is published by the force
+//        pass); P.pres is written here.  Without WORK density goes to P.rho.
+template <typename T, bool SHEP, bool DENS, bool WORK>
+__global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P, BndV<T> B,
+                                                            GridV<T> gs, GridV<T> gb,
+                                                            WorkV<T> w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const V4<T> me = gs.s[k];
+    const int i = gs.ids[k];
+    const T sup = p.support, ss = sup * sup, h = p.h;
+    T flt = (T)0, rho = (T)0;
+    query_runs(gs, me.x, me.y, me.z, sup, [&](int a, int b) {
+        for (int s = a; s < b; ++s) {
+            const V4<T> q = gs.s[s];
+            const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+            const T td = dx * dx + dy * dy + dz * dz;
+            if (td < ss) {
+                const T wv = wendland_w(msqrt(td), h);
+                if (SHEP) flt = flt + (q.w / P.rho[gs.ids[s]]) * wv;
+                if (DENS) rho = rho + q.w * wv;
+            }
+        }
+    });
+    query_runs(gb, me.x, me.y, me.z, sup, [&](int a, int b) {
+        for (int s = a; s < b; ++s) {
+            const V4<T> q = gb.s[s];
+            const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+            const T td = dx * dx + dy * dy + dz * dz;
+            if (td < ss) {
+                const T wv = wendland_w(msqrt(td), h);
+                if (SHEP) flt = flt + (q.w / B.rho[gb.ids[s]]) * wv;
+                if (DENS) rho = rho + q.w * wv;
+            }
+        }
+    });
+    if (SHEP) P.flt[i] = flt;
+    if (DENS) {
+        const T r = rho / (SHEP ? flt : P.flt[i]);
+        if (WORK) {
+            const T pr = tait(r, p.rho0, p.c0, p.gamma);
+            w.fl_rho[k] = r;
+            w.fl_pres[k] = pr;
+            P.pres[i] = pr;
+            w.fl_vel[k] = V4<T>{P.vel[3 * i], P.vel[3 * i + 1], P.vel[3 * i + 2], (T)0};
+        } else {
+            P.rho[i] = r;
+        }
+    }
+}
+
+// Wall subjects: raw density (KERNEL_rho.py:87-126); with WORK also the EOS
+// and the sorted copies the force pass reads.
+template <typename T, bool WORK>
+__global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, BndV<T> B,
+                                                           GridV<T> gs, GridV<T> gb, WorkV<T> w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gb.n) return;
+    const V4<T> me = gb.s[k];
+    const int bi = gb.ids[k];
+    const T sup = p.support, ss = sup * sup, h = p.h;
+    T rho = (T)0;
+    auto acc = [&](const GridV<T> &g) {
+        query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
+            for (int s = a; s < b; ++s) {
+                const V4<T> q = g.s[s];
+                const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+                const T td = dx * dx + dy * dy + dz * dz;
+                if (td < ss) rho = rho + q.w * wendland_w(msqrt(td), h);
+            }
+        });
+    };
+    acc(gs);
+    acc(gb);
+    B.rho[bi] = rho;
+    if (WORK) {
+        const T pr = tait(rho, p.rho0, p.c0, p.gamma);
+        B.pres[bi] = pr;
+        w.wl_rho[k] = rho;
+        w.wl_pres[k] = pr;
+        w.wl_vel[k] = V4<T>{B.vel[3 * bi], B.vel[3 * bi + 1], B.vel[3 * bi + 2], (T)0};
+    }
+}
+
+template <typename T>
+__global__ void k_pres(Prm<T> p, const T *rho, T *pres, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pres[i] = tait(rho[i], p.rho0, p.c0, p.gamma);
+}
+
+// Fluid subjects: acceleration (KERNEL_force.py:29-71) and/or the SPH-scale
+// -grad p of Kernel_prep_sphdem (KERNEL_SPHDEM_interaction.py:98-113).  Both
+// loops keep 1e-12 < r^2 < support^2 with the same support.
+//  PUBLISH: write the sorted density back to P.rho (fused step).
+//  PGF also leaves (pgf, rho) and m/rho in sorted order for the coupling.
+template <typename T, bool ACC, bool PGF, bool PUBLISH>
+__global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, BndV<T> B,
+                                                          GridV<T> gs, GridV<T> gb, WorkV<T> w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const V4<T> me = gs.s[k];
+    const int i = gs.ids[k];
+    const T rhoi = w.fl_rho[k], presi = w.fl_pres[k];
+    const V3<T> vi = xyz(w.fl_vel[k]);
+    const T sup = p.support, ss = sup * sup, h = p.h, r2min = (T)1.0e-12;
+    const T ai = presi / (rhoi * rhoi);
+    const T two_mu = (T)2.0 * p.mu, reg = (T)0.01 * h * h;
+    V3<T> acc = mk3<T>(0, 0, 0), pgf = mk3<T>(0, 0, 0);
+    auto body = [&](const GridV<T> &g, const T *nrho, const T *npres, const V4<T> *nvel) {
+        query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
+            for (int s = a; s < b; ++s) {
+                const V4<T> q = g.s[s];
+                const V3<T> rij = mk3(me.x - q.x, me.y - q.y, me.z - q.z);
+                const T td = dot(rij, rij);
+                if (td > r2min && td < ss) {
+                    const T r = msqrt(td);
+                    const T dwv = wendland_dw(r, h);
+                    const T rhoj = nrho[s], presj = npres[s];
+                    if (ACC) {
+                        const V3<T> dwij = dwv * (rij / r);
+                        acc = acc - (q.w * (ai + presj / (rhoj * rhoj))) * dwij;
+                        const T visc = two_mu * dot(rij, dwij) / (rhoi * rhoj * (td + reg));
+                        acc = acc + (q.w * visc) * (vi - xyz(nvel[s]));
+                    }
+                    if (PGF) {
+                        const V3<T> dwp = (dwv * rij) / r;
+                        pgf = pgf - ((q.w / rhoj) * (presj - presi)) * dwp;
+                    }
+                }
+            }
+        });
+    };
+    body(gs, w.fl_rho, w.fl_pres, w.fl_vel);
+    body(gb, w.wl_rho, w.wl_pres, w.wl_vel);
+    if (ACC) st3(P.acc, i, acc + mk3<T>(0, 0, -p.g));
+    if (PUBLISH) P.rho[i] = rhoi;
+    if (PGF) {
+        st3(P.pgf, i, pgf);
+        w.fl_pgf[k] = V4<T>{pgf.x, pgf.y, pgf.z, rhoi};
+        w.fl_V[k] = me.w / rhoi;
+    }
+}
+
+// Sorted working copies of the current fluid / wall fields, for the
+// single-kernel entry points that start from the particle arrays.
+template <typename T>
+__global__ void k_gather_fluid(SphV<T> P, GridV<T> gs, WorkV<T> w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const int i = gs.ids[k];
+    const T r = P.rho[i];
+    w.fl_rho[k] = r;
+    w.fl_pres[k] = P.pres[i];
+    w.fl_vel[k] = V4<T>{P.vel[3 * i], P.vel[3 * i + 1], P.vel[3 * i + 2], (T)0};
+    w.fl_V[k] = P.m[i] / r;
+    w.fl_pgf[k] = V4<T>{P.pgf[3 * i], P.pgf[3 * i + 1], P.pgf[3 * i + 2], r};
+}
+
+template <typename T>
+__global__ void k_gather_wall(BndV<T> B, GridV<T> gb, WorkV<T> w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gb.n) return;
+    const int bi = gb.ids[k];
+    w.wl_rho[k] = B.rho[bi];
+    w.wl_pres[k] = B.pres[bi];
+    w.wl_vel[k] = V4<T>{B.vel[3 * bi], B.vel[3 * bi + 1], B.vel[3 * bi + 2], (T)0};
+}
+
+// KERNEL_step.py:6-22 (tid order; elementwise).
+template <typename T>
+__global__ void k_step_sph(SphV<T> P, T dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const V3<T> v = ld3(P.vel, i) + ld3(P.acc, i) * dt;
+    st3(P.vel, i, v);
+    st3(P.pos, i, ld3(P.pos, i) + v * dt);
+}
+
+template <typename T>
+__global__ void k_wendland(const T *r, int n, T h, T *w, T *dw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    w[i] = wendland_w(r[i], h);
+    dw[i] = wendland_dw(r[i], h);
+}
+
+// ---------------------------------------------------------------- host side
+int check_sph_pair(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+                   const sphdem_grid_t *gs, const sphdem_grid_t *gb) {
+    SPHDEM_REQUIRE(p && P && B && gs && gb, "NULL argument");
+    SPHDEM_REQUIRE(p->precision == 4 || p->precision == 8, "precision must be 4 or 8");
+    SPHDEM_REQUIRE(gs->precision == p->precision && gb->precision == p->precision,
+                   "grid precision differs from the particle precision");
+    SPHDEM_REQUIRE(gs->n == P->n, "fluid grid was built from a different number of points");
+    SPHDEM_REQUIRE(gb->n == B->n, "wall grid was built from a different number of points");
+    SPHDEM_REQUIRE(P->n == 0 || (P->pos && P->vel && P->rho && P->pres && P->acc && P->m && P->flt),
+                   "fluid field pointer is NULL");
+    SPHDEM_REQUIRE(B->n == 0 || (B->pos && B->vel && B->rho && B->pres && B->m),
+                   "wall field pointer is NULL");
+    SPHDEM_REQUIRE(gs->n == 0 || (gs->point_ids && gs->sorted), "fluid grid is not built");
+    SPHDEM_REQUIRE(gb->n == 0 || (gb->point_ids && gb->sorted), "wall grid is not built");
+    return SPHDEM_OK;
+}
+
+int check_work(const sphdem_work_t *w, int nf, int nb) {
+    SPHDEM_REQUIRE(w != nullptr, "work is NULL");
+    SPHDEM_REQUIRE(w->cap_f >= nf && w->cap_b >= nb, "work arrays are smaller than the particle sets");
+    SPHDEM_REQUIRE(nf == 0 || (w->fl_rho && w->fl_pres && w->fl_vel && w->fl_V && w->fl_pgf),
+                   "fluid work pointer is NULL");
+    SPHDEM_REQUIRE(nb == 0 || (w->wl_rho && w->wl_pres && w->wl_vel), "wall work pointer is NULL");
+    return SPHDEM_OK;
+}
+
+template <typename T, bool SHEP, bool DENS, bool WORK>
+static void launch_fluid_density(const sphdem_params_t *p, const sphdem_sph_t *P,
+                                 const sphdem_bnd_t *B, const sphdem_grid_t *gs,
+                                 const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
+    if (gs->n == 0) return;
+    WorkV<T> wv{};
+    if (w) wv = view<T>(w);
+    hipLaunchKernelGGL((k_fluid_density<T, SHEP, DENS, WORK>), dim3(blocks_for(gs->n, kThreads)),
+                       dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
+                       view<T>(gb), wv);
+}
+
+template <typename T, bool WORK>
+static void launch_wall_density(const sphdem_params_t *p, const sphdem_sph_t *P,
+                                const sphdem_bnd_t *B, const sphdem_grid_t *gs,
+                                const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
+    if (gb->n == 0) return;
+    WorkV<T> wv{};
+    if (w) wv = view<T>(w);
+    hipLaunchKernelGGL((k_wall_density<T, WORK>), dim3(blocks_for(gb->n, kThreads)), dim3(kThreads),
+                       0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb), wv);
+}
+
+template <typename T, bool ACC, bool PGF, bool PUBLISH>
+static void launch_fluid_force(const sphdem_params_t *p, const sphdem_sph_t *P,
+                               const sphdem_bnd_t *B, const sphdem_grid_t *gs,
+                               const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
+    if (gs->n == 0) return;
+    hipLaunchKernelGGL((k_fluid_force<T, ACC, PGF, PUBLISH>), dim3(blocks_for(gs->n, kThreads)),
+                       dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
+                       view<T>(gb), view<T>(w));
+}
+
+template <typename T>
+void launch_gather(const sphdem_sph_t *P, const sphdem_bnd_t *B, const sphdem_grid_t *gs,
+                   const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
+    if (P && gs && gs->n > 0)
+        hipLaunchKernelGGL(k_gather_fluid<T>, dim3(blocks_for(gs->n, kThreads)), dim3(kThreads), 0,
+                           st, view<T>(P), view<T>(gs), view<T>(w));
+    if (B && gb && gb->n > 0)
+        hipLaunchKernelGGL(k_gather_wall<T>, dim3(blocks_for(gb->n, kThreads)), dim3(kThreads), 0,
+                           st, view<T>(B), view<T>(gb), view<T>(w));
+}
+template void launch_gather<float>(const sphdem_sph_t *, const sphdem_bnd_t *,
+                                   const sphdem_grid_t *, const sphdem_grid_t *,
+                                   const sphdem_work_t *, hipStream_t);
+template void launch_gather<double>(const sphdem_sph_t *, const sphdem_bnd_t *,
+                                    const sphdem_grid_t *, const sphdem_grid_t *,
+                                    const sphdem_work_t *, hipStream_t);
+
+template <typename T>
+void launch_step_sph(const sphdem_params_t *p, const sphdem_sph_t *P, hipStream_t st) {
+    if (P->n == 0) return;
+    hipLaunchKernelGGL(k_step_sph<T>, dim3(blocks_for(P->n, kThreads)), dim3(kThreads), 0, st,
+                       view<T>(P), (T)p->dt);
+}
+
+// SPH part of one step (Simulation.py:17-91), fused: P1 fluid Shepard +
+// density + EOS, P1b wall density + EOS, P2 force (+ -grad p when coupled).
+template <typename T>
+int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+              const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step, bool coupled,
+              const sphdem_work_t *w, hipStream_t st) {
+    const bool shep = step == 0 || (p->shepard_step > 0 && step % p->shepard_step == 0);
+    if (shep)
+        launch_fluid_density<T, true, true, true>(p, P, B, gs, gb, w, st);
+    else
+        launch_fluid_density<T, false, true, true>(p, P, B, gs, gb, w, st);
+    launch_wall_density<T, true>(p, P, B, gs, gb, w, st);
+    if (coupled)
+        launch_fluid_force<T, true, true, true>(p, P, B, gs, gb, w, st);
+    else
+        launch_fluid_force<T, true, false, true>(p, P, B, gs, gb, w, st);
+    return check_launch("sph_stage");
+}
+template int sph_stage<float>(const sphdem_params_t *, const sphdem_sph_t *, const sphdem_bnd_t *,
+                              const sphdem_grid_t *, const sphdem_grid_t *, int, bool,
+                              const sphdem_work_t *, hipStream_t);
+template int sph_stage<double>(const sphdem_params_t *, const sphdem_sph_t *, const sphdem_bnd_t *,
+                               const sphdem_grid_t *, const sphdem_grid_t *, int, bool,
+                               const sphdem_work_t *, hipStream_t);
+
+// SPH-scale -grad p alone, from the particle arrays (prep_sphdem entry).
+template <typename T>
+void launch_pgf_only(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+                     const sphdem_grid_t *gs, const sphdem_grid_t *gb, const sphdem_work_t *w,
+                     hipStream_t st) {
+    launch_gather<T>(P, B, gs, gb, w, st);
+    launch_fluid_force<T, false, true, false>(p, P, B, gs, gb, w, st);
+}
+template void launch_pgf_only<float>(const sphdem_params_t *, const sphdem_sph_t *,
+                                     const sphdem_bnd_t *, const sphdem_grid_t *,
+                                     const sphdem_grid_t *, const sphdem_work_t *, hipStream_t);
+template void launch_pgf_only<double>(const sphdem_params_t *, const sphdem_sph_t *,
+                                      const sphdem_bnd_t *, const sphdem_grid_t *,
+                                      const sphdem_grid_t *, const sphdem_work_t *, hipStream_t);
+
+}  // namespace sphdem
+
+using namespace sphdem;
+
+#define DISPATCH(prec, CALL)                 \
+    do {                                     \
+        if ((prec) == 4) {                   \
+            using T = float;                 \
+            CALL;                            \
+        } else {                             \
+            using T = double;                \
+            CALL;                            \
+        }                                    \
+    } while (0)
+
+extern "C" {
+
+int sphdem_wendland(int precision, const void *r, int n, double h, void *w, void *dw,
+                    void *stream) {
+    SPHDEM_REQUIRE(precision == 4 || precision == 8, "precision must be 4 or 8");
+    SPHDEM_REQUIRE(n >= 0 && (n == 0 || (r && w && dw)), "invalid Wendland buffers");
+    if (n == 0) return SPHDEM_OK;
+    DISPATCH(precision, hipLaunchKernelGGL(k_wendland<T>, dim3(blocks_for(n, kThreads)),
+                                           dim3(kThreads), 0, as_stream(stream), (const T *)r, n,
+                                           (T)h, (T *)w, (T *)dw));
+    return check_launch("wendland");
+}
+
+int sphdem_shepard_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    DISPATCH(p->precision, (launch_fluid_density<T, true, false, false>(p, P, B, gs, gb, nullptr,
+                                                                         as_stream(stream))));
+    return check_launch("shepard_sph");
+}
+
+int sphdem_density_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    DISPATCH(p->precision, (launch_fluid_density<T, false, true, false>(p, P, B, gs, gb, nullptr,
+                                                                         as_stream(stream))));
+    return check_launch("density_sph");
+}
+
+int sphdem_density_bnd(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                       const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    DISPATCH(p->precision,
+             (launch_wall_density<T, false>(p, P, B, gs, gb, nullptr, as_stream(stream))));
+    return check_launch("density_bnd");
+}
+
+int sphdem_pres(const sphdem_params_t *p, const void *rho, void *pres, int n, void *stream) {
+    SPHDEM_REQUIRE(p && (p->precision == 4 || p->precision == 8), "invalid params");
+    SPHDEM_REQUIRE(n >= 0 && (n == 0 || (rho && pres)), "invalid pressure buffers");
+    if (n == 0) return SPHDEM_OK;
+    DISPATCH(p->precision, hipLaunchKernelGGL(k_pres<T>, dim3(blocks_for(n, kThreads)),
+                                              dim3(kThreads), 0, as_stream(stream), prm<T>(p),
+                                              (const T *)rho, (T *)pres, n));
+    return check_launch("pres");
+}
+
+int sphdem_force_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                     const sphdem_grid_t *gs, const sphdem_grid_t *gb, const sphdem_work_t *w,
+                     void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    rc = check_work(w, P->n, B->n);
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    DISPATCH(p->precision, (launch_gather<T>(P, B, gs, gb, w, st),
+                            launch_fluid_force<T, true, false, false>(p, P, B, gs, gb, w, st)));
+    return check_launch("force_sph");
+}
+
+int sphdem_step_sph(const sphdem_params_t *p, sphdem_sph_t *P, void *stream) {
+    SPHDEM_REQUIRE(p && P && (p->precision == 4 || p->precision == 8), "invalid arguments");
+    SPHDEM_REQUIRE(P->n == 0 || (P->pos && P->vel && P->acc), "fluid field pointer is NULL");
+    DISPATCH(p->precision, launch_step_sph<T>(p, P, as_stream(stream)));
+    return check_launch("step_sph");
+}
+
+int sphdem_sph_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                    const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step, int integrate,
+                    const sphdem_work_t *w, void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    rc = check_work(w, P->n, B->n);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(step >= 0, "step must be nonnegative");
+    hipStream_t st = as_stream(stream);
+    DISPATCH(p->precision, rc = sph_stage<T>(p, P, B, gs, gb, step, false, w, st));
+    if (rc) return rc;
+    if (integrate) DISPATCH(p->precision, launch_step_sph<T>(p, P, st));
+    return check_launch("sph_step");
+}
+
+}  // extern "C"

@@ -1,0 +1,419 @@
+"""Initial-condition generators and the particle-file writer.
+
+Mirrors input_gen/config.py:10-363 (GenerationConfig, CoupledGenerationConfig),
+input_gen/gen_ptl.py:8-101 (DamPtlGeneration), input_gen/gen_dem.py:9-125
+(DEMPtlGeneration) and input_gen/generate.py:20-44 (write_particle_file,
+generate_files).  The lattice formulas, float32 rounding and the "%.9g"
+tab-separated format are kept, so the reference's shipped input files are
+reproduced byte for byte.  Seeded random sequential addition of DEM
+spheres (`rsa_spheres`) is an addition for the larger workloads.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from .input_reader import INPUT_FIELDS, column_indices
+from .struct import BNDptl, DEMBNDptl, DEMptl, SPHptl
+
+
+@dataclass
+class GenerationConfig:
+    """Preprocessor settings (input_gen/config.py:10-175)."""
+
+    device: str = "cuda:0"
+    dx: float = 0.04
+    tank_width: float = 2.0
+    tank_height: float = 1.0
+    tank_depth: float = 0.4
+    fluid_width: float = 0.5
+    fluid_height: float = 0.5
+    fluid_depth: float = 0.4
+    fluid_origin_x: float = 0.0
+    fluid_origin_y: float = 0.0
+    fluid_origin_z: float = 0.0
+    bnd_layer: int = 3
+    h: float | None = None
+    support: float | None = None
+    rho0: float = 1.0
+    gamma: float = 7.0
+    c0: float = 31.3209
+    mu: float = 0.05
+    g: float = 9.81
+    h_factor: float = 1.3
+    shepard_step: int = 20
+    dt: float = 2.0e-3
+    n_steps: int = 5000
+    output_step: int = 100
+    gif_save: bool = True
+    output_dir: str = "result/3d"
+    animation_dir: str = "animation/3d"
+    grid_slice: int = 64
+    dem_enable: bool = False
+    dem_radius: float = 0.025
+    dem_rho: float = 2500.0
+    dem_K: float = 2.0e4
+    dem_eta: float = 4.0
+    dem_mu: float = 0.3
+    dem_inertia_factor: float = 0.4
+    dem_nx: int = 5
+    dem_ny: int = 2
+    dem_nz: int = 2
+    dem_spacing: float = 0.065
+    dem_origin_x: float = 0.12
+    dem_origin_y: float = 0.1675
+    dem_origin_z: float = 0.62
+    dem_vel_x: float = 0.0
+    dem_vel_y: float = 0.0
+    dem_vel_z: float = 0.0
+    dem_omega_x: float = 0.0
+    dem_omega_y: float = 0.0
+    dem_omega_z: float = 0.0
+    dem_bnd_radius: float = 0.01
+    dem_bnd_rho: float = 2500.0
+    dem_bnd_spacing: float = 0.02
+    dem_h: float = 0.08
+    dem_porosity_min: float = 0.05
+    dem_porosity_max: float = 1.0
+    dem_dt_safety: float = 0.1
+
+    def __post_init__(self) -> None:
+        if self.h is None:
+            self.h = self.h_factor * self.dx
+        if self.support is None:
+            self.support = 2.0 * self.h
+
+    @property
+    def dem_volume(self) -> float:
+        return (4.0 / 3.0) * math.pi * self.dem_radius ** 3
+
+    @property
+    def dem_mass(self) -> float:
+        return self.dem_rho * self.dem_volume
+
+    @property
+    def dem_inertia(self) -> float:
+        return self.dem_inertia_factor * self.dem_mass * self.dem_radius ** 2
+
+    @property
+    def dem_bnd_volume(self) -> float:
+        return (4.0 / 3.0) * math.pi * self.dem_bnd_radius ** 3
+
+    @property
+    def dem_bnd_mass(self) -> float:
+        return self.dem_bnd_rho * self.dem_bnd_volume
+
+    @property
+    def dem_bnd_inertia(self) -> float:
+        return self.dem_inertia_factor * self.dem_bnd_mass * self.dem_bnd_radius ** 2
+
+    @property
+    def dem_support(self) -> float:
+        return 2.0 * self.dem_h
+
+    def validate_sph(self) -> None:
+        """input_gen/config.py:227-263: domain, lattice and dt checks."""
+        for name in ("dx", "h", "support", "tank_width", "tank_height", "tank_depth",
+                     "fluid_width", "fluid_height", "fluid_depth", "rho0", "c0", "gamma", "dt"):
+            value = getattr(self, name)
+            if not math.isfinite(value) or value <= 0.0:
+                raise ValueError(f"{name} must be finite and positive")
+        if not math.isclose(self.support, 2.0 * self.h):
+            raise ValueError("Wendland support must equal 2*h")
+        if self.dt > 0.25 * self.h / self.c0:
+            raise ValueError("dt exceeds the initial SPH acoustic limit 0.25*h/c0")
+        for name in ("g", "mu"):
+            if not math.isfinite(getattr(self, name)) or getattr(self, name) < 0.0:
+                raise ValueError(f"{name} must be finite and nonnegative")
+        for name in ("bnd_layer", "grid_slice"):
+            if not isinstance(getattr(self, name), int) or getattr(self, name) < 1:
+                raise ValueError(f"{name} must be a positive integer")
+        for name in ("n_steps", "output_step", "shepard_step"):
+            if not isinstance(getattr(self, name), int) or getattr(self, name) < 0:
+                raise ValueError(f"{name} must be a nonnegative integer")
+        for name in ("tank_width", "tank_height", "tank_depth"):
+            cells = getattr(self, name) / self.dx
+            if not math.isclose(cells, round(cells), abs_tol=1.0e-8):
+                raise ValueError(f"{name} must be an integer multiple of dx")
+        for axis, extent, tank in (("x", self.fluid_width, self.tank_width),
+                                   ("y", self.fluid_depth, self.tank_depth),
+                                   ("z", self.fluid_height, self.tank_height)):
+            origin = getattr(self, "fluid_origin_" + axis)
+            if (not math.isfinite(origin) or origin < 0.0 or origin + extent > tank + 1.0e-12
+                    or round(extent / self.dx) < 1):
+                raise ValueError(f"initial fluid block must fit the tank along {axis}")
+
+    def validate_dem(self) -> None:
+        """input_gen/config.py:307-353: DEM material, spacing and contact dt."""
+        if not self.dem_enable:
+            return
+        for name in ("dem_radius", "dem_rho", "dem_K", "dem_inertia_factor",
+                     "dem_bnd_radius", "dem_bnd_rho", "dem_bnd_spacing",
+                     "dem_h", "dem_dt_safety", "dt"):
+            value = getattr(self, name)
+            if not math.isfinite(value) or value <= 0.0:
+                raise ValueError(f"{name} must be finite and positive")
+        if (not math.isfinite(self.dem_eta) or not math.isfinite(self.dem_mu)
+                or self.dem_eta < 0.0 or self.dem_mu < 0.0):
+            raise ValueError("dem_eta and dem_mu must be nonnegative")
+        if any(not isinstance(n, int) or n < 1 for n in (self.dem_nx, self.dem_ny, self.dem_nz)):
+            raise ValueError("dem_nx, dem_ny and dem_nz must be positive integers")
+        if not math.isfinite(self.dem_spacing) or self.dem_spacing < 2.0 * self.dem_radius:
+            raise ValueError("initial DEM particles must not overlap")
+        if self.dem_bnd_spacing > 2.0 * self.dem_bnd_radius:
+            raise ValueError("DEM boundary spacing must not exceed the boundary diameter")
+        if self.dem_bnd_spacing / math.sqrt(2.0) >= self.dem_radius + self.dem_bnd_radius:
+            raise ValueError("DEM boundary pores must be smaller than the moving spheres")
+        for name in ("dem_origin_x", "dem_origin_y", "dem_origin_z", "dem_vel_x",
+                     "dem_vel_y", "dem_vel_z", "dem_omega_x", "dem_omega_y", "dem_omega_z"):
+            if not math.isfinite(getattr(self, name)):
+                raise ValueError(f"{name} must be finite")
+        if not 0.0 < self.dem_porosity_min <= self.dem_porosity_max <= 1.0:
+            raise ValueError("DEM porosity bounds must satisfy 0 < min <= max <= 1")
+        mass_eff = 0.5 * self.dem_mass
+        dt_contact = self.dem_dt_safety * min(math.sqrt(mass_eff / self.dem_K),
+                                              mass_eff / max(self.dem_eta, 1.0e-20))
+        if self.dt > dt_contact:
+            raise ValueError(f"dt exceeds DEM contact limit {dt_contact:.6g} s")
+
+
+@dataclass
+class CoupledGenerationConfig(GenerationConfig):
+    """The 3-D SPH-DEM dam-break example (input_gen/config.py:355-363)."""
+
+    dx: float = 0.02
+    rho0: float = 1000.0
+    dt: float = 1.0e-4
+    n_steps: int = 9000
+    output_step: int = 90
+    dem_enable: bool = True
+
+
+def _f32(a) -> torch.Tensor:
+    """float64 numpy -> float32 tensor, rounding to nearest as Warp arrays do."""
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64)).astype(np.float32))
+
+
+class DamPtlGeneration:
+    """Fluid block and 3-layer dummy walls of the dam break (gen_ptl.py:8-101)."""
+
+    def __init__(self, solv: GenerationConfig) -> None:
+        self.solv = solv
+        self.dx = solv.dx
+        self.bnd_layer = solv.bnd_layer
+
+    def fluid_particle(self) -> np.ndarray:
+        s, dx = self.solv, self.dx
+        nx, ny, nz = (int(round(s.fluid_width / dx)), int(round(s.fluid_depth / dx)),
+                      int(round(s.fluid_height / dx)))
+        gx, gy, gz = np.meshgrid(s.fluid_origin_x + np.arange(nx) * dx,
+                                 s.fluid_origin_y + np.arange(ny) * dx,
+                                 s.fluid_origin_z + np.arange(nz) * dx, indexing="ij")
+        return np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1)
+
+    def boundary_particle(self) -> np.ndarray:
+        """Bottom and four side walls, bnd_layer thick, open top; each lattice
+        site once (gen_ptl.py:39-61)."""
+        s, dx, L = self.solv, self.dx, self.bnd_layer
+        nx, ny, nz = (int(round(s.tank_width / dx)), int(round(s.tank_depth / dx)),
+                      int(round(s.tank_height / dx)))
+        gx, gy, gz = np.meshgrid(np.arange(-L, nx + L), np.arange(-L, ny + L),
+                                 np.arange(-L, nz + L), indexing="ij")
+        wall = (gz < 0) | (gx < 0) | (gx >= nx) | (gy < 0) | (gy >= ny)
+        return np.stack([gx[wall], gy[wall], gz[wall]], axis=1) * dx
+
+    def build(self) -> tuple[SPHptl, BNDptl]:
+        s = self.solv
+        s.validate_sph()
+        fpos, bpos = self.fluid_particle(), self.boundary_particle()
+        mass = s.rho0 * s.dx ** 3
+        dev = s.device
+        P = SPHptl.allocate(len(fpos), device="cpu")
+        P.pos = _f32(fpos)
+        P.rho_raw.fill_(s.rho0)
+        P.rho.fill_(s.rho0)
+        P.m.fill_(mass)
+        P.porosity.fill_(1.0)
+        B = BNDptl.allocate(len(bpos), device="cpu")
+        B.pos = _f32(bpos)
+        B.rho_raw.fill_(s.rho0)
+        B.rho.fill_(s.rho0)
+        B.m.fill_(mass)
+        return P.to(dev), B.to(dev)
+
+
+class DEMPtlGeneration:
+    """Moving DEM lattice and fixed DEM wall spheres (gen_dem.py:9-125)."""
+
+    def __init__(self, solv: GenerationConfig) -> None:
+        self.solv = solv
+
+    def dem_particle(self) -> np.ndarray:
+        s = self.solv
+        gx, gy, gz = np.meshgrid(s.dem_origin_x + np.arange(s.dem_nx) * s.dem_spacing,
+                                 s.dem_origin_y + np.arange(s.dem_ny) * s.dem_spacing,
+                                 s.dem_origin_z + np.arange(s.dem_nz) * s.dem_spacing,
+                                 indexing="ij")
+        pos = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1)
+        check_dem_inside(pos, s.dem_radius, s)
+        return pos
+
+    def boundary_particle(self) -> np.ndarray:
+        """Wall spheres one radius outside the tank, spacing <= dem_bnd_spacing,
+        shell only, no ceiling (gen_dem.py:45-67)."""
+        s = self.solv
+        r = s.dem_bnd_radius
+        nx = int(np.ceil((s.tank_width + 2.0 * r) / s.dem_bnd_spacing))
+        ny = int(np.ceil((s.tank_depth + 2.0 * r) / s.dem_bnd_spacing))
+        nz = int(np.ceil((s.tank_height + r) / s.dem_bnd_spacing))
+        x = np.linspace(-r, s.tank_width + r, nx + 1)
+        y = np.linspace(-r, s.tank_depth + r, ny + 1)
+        z = np.linspace(-r, s.tank_height, nz + 1)
+        gx, gy, gz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                                 indexing="ij")
+        wall = (gz == 0) | (gx == 0) | (gx == nx) | (gy == 0) | (gy == ny)
+        return np.stack([x[gx[wall]], y[gy[wall]], z[gz[wall]]], axis=1)
+
+    def build(self, dem_pos: np.ndarray | None = None,
+              dem_radius: np.ndarray | None = None) -> tuple[DEMptl, DEMBNDptl]:
+        """Allocate DEM and DEM-wall structs.  `dem_pos`/`dem_radius` replace
+        the lattice with given centres and (per-sphere) radii."""
+        s = self.solv
+        s.validate_dem()
+        dpos = self.dem_particle() if dem_pos is None else np.asarray(dem_pos, np.float64)
+        bpos = self.boundary_particle()
+        nd, nb = len(dpos), len(bpos)
+        D = DEMptl.allocate(nd, device="cpu")
+        W = DEMBNDptl.allocate(nb, device="cpu")
+        D.pos, W.pos = _f32(dpos), _f32(bpos)
+        if dem_radius is None:
+            D.radius.fill_(s.dem_radius)
+            D.rho.fill_(s.dem_rho)
+            D.m.fill_(s.dem_mass)
+            D.volume.fill_(s.dem_volume)
+            D.inertia.fill_(s.dem_inertia)
+        else:
+            r = np.broadcast_to(np.asarray(dem_radius, np.float64), (nd,))
+            vol = (4.0 / 3.0) * math.pi * r ** 3
+            mass = s.dem_rho * vol
+            D.radius, D.rho = _f32(r), _f32(np.full(nd, s.dem_rho))
+            D.m, D.volume = _f32(mass), _f32(vol)
+            D.inertia = _f32(s.dem_inertia_factor * mass * r ** 2)
+        for T in (D, W):
+            T.K.fill_(s.dem_K)
+            T.eta.fill_(s.dem_eta)
+            T.mu.fill_(s.dem_mu)
+        W.radius.fill_(s.dem_bnd_radius)
+        W.rho.fill_(s.dem_bnd_rho)
+        W.m.fill_(s.dem_bnd_mass)
+        W.volume.fill_(s.dem_bnd_volume)
+        W.inertia.fill_(s.dem_bnd_inertia)
+        D.h.fill_(s.dem_h)
+        D.vel[:] = torch.tensor([s.dem_vel_x, s.dem_vel_y, s.dem_vel_z], dtype=torch.float32)
+        D.omega[:] = torch.tensor([s.dem_omega_x, s.dem_omega_y, s.dem_omega_z],
+                                  dtype=torch.float32)
+        D.porosity.fill_(1.0)
+        return D.to(s.device), W.to(s.device)
+
+
+def check_dem_inside(pos: np.ndarray, radius, s: GenerationConfig) -> None:
+    """Sphere extents inside the five closed tank sides (gen_dem.py:32-40)."""
+    r = np.broadcast_to(np.asarray(radius, np.float64), (len(pos),))
+    if len(pos) and (np.min(pos[:, 0] - r) < 0 or np.max(pos[:, 0] + r) > s.tank_width
+                     or np.min(pos[:, 1] - r) < 0 or np.max(pos[:, 1] + r) > s.tank_depth
+                     or np.min(pos[:, 2] - r) < 0):
+        raise ValueError("initial DEM spheres must lie inside the tank walls")
+
+
+def rsa_spheres(n: int, radius, box_lo, box_hi, seed: int, attempts: int = 200_000,
+                grow_z: float = 0.0, max_grow: int = 200, r_max: float | None = None):
+    """Seeded random sequential addition of non-overlapping spheres.
+
+    Centres are drawn uniformly in [box_lo, box_hi]; a candidate is kept when
+    its centre distance to every kept sphere is at least the sum of radii.
+    `radius` is a scalar or a callable rng -> radius (then r_max bounds it).  If `attempts` draws
+    do not place n spheres and grow_z > 0, the box top is raised by grow_z
+    and drawing continues (at most max_grow times).  Returns (centres,
+    radii, box_hi_used) with float64 arrays in placement order.
+    """
+    rng = np.random.default_rng(seed)
+    lo = np.asarray(box_lo, np.float64)
+    hi = np.asarray(box_hi, np.float64).copy()
+    pos = np.zeros((n, 3))
+    rad = np.zeros(n)
+    k = 0
+    if callable(radius) and r_max is None:
+        raise ValueError("r_max is required with a radius sampler")
+    cell = 2.0 * (float(radius) if r_max is None else float(r_max))
+    buckets: dict = {}
+    for _ in range(max_grow + 1):
+        for _ in range(attempts):
+            if k == n:
+                break
+            r = float(radius(rng)) if callable(radius) else float(radius)
+            c = lo + rng.random(3) * (hi - lo)
+            key = tuple((c // cell).astype(int))
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        for j in buckets.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
+                            if np.sum((pos[j] - c) ** 2) < (rad[j] + r) ** 2:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                pos[k], rad[k] = c, r
+                buckets.setdefault(key, []).append(k)
+                k += 1
+        if k == n or grow_z <= 0.0:
+            break
+        hi[2] += grow_z
+    return pos[:k], rad[:k], hi
+
+
+def write_particle_file(path: str | Path, kind: str, particle) -> Path:
+    """Write the input columns of one struct with integer column IDs
+    (input_gen/generate.py:20-31): '%.9g', tab separated, no comment mark."""
+    destination = Path(path)
+    destination.parent.mkdir(parents=True, exist_ok=True)
+    n = particle.pos.shape[0]
+    cols = [getattr(particle, name).detach().cpu().numpy().reshape(n, width)
+            for name, width in INPUT_FIELDS[kind]]
+    header = "\t".join(str(i) for ids in column_indices(kind).values() for i in ids)
+    np.savetxt(destination, np.concatenate(cols, axis=1), fmt="%.9g", delimiter="\t",
+               header=header, comments="", encoding="utf-8")
+    return destination
+
+
+def generate_files(output_dir: str | Path,
+                   coupled: CoupledGenerationConfig | None = None) -> dict:
+    """Write input_{SPH,BND,DEM,DEMBND}.txt of the dam-break case
+    (input_gen/generate.py:34-44, without the EISPH cavity)."""
+    coupled = CoupledGenerationConfig(device="cpu") if coupled is None else coupled
+    particles = dict(zip(("SPH", "BND"), DamPtlGeneration(coupled).build()))
+    particles.update(zip(("DEM", "DEMBND"), DEMPtlGeneration(coupled).build()))
+    return {kind: write_particle_file(Path(output_dir) / f"input_{kind}.txt", kind, p)
+            for kind, p in particles.items()}
+
+
+def main() -> None:
+    import argparse
+    parser = argparse.ArgumentParser(description="Generate the SPH/DEM dam-break input files")
+    parser.add_argument("--output-dir", type=Path, default=Path("input_file"))
+    args = parser.parse_args()
+    for kind, path in generate_files(args.output_dir).items():
+        print(f"[{kind}] {path}")
+
+
+if __name__ == "__main__":
+    main()
