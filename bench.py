@@ -1,0 +1,173 @@
+"""Benchmark: SPH particle-steps/s of the coupled WCSPH-DEM step on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+
+Workload (default c2): 1,562,500 fluid particles (dx = 0.004), 1,080,816
+dummy-wall particles, 2,000 DEM spheres, 14,688 DEM-wall spheres,
+dt = 2.5e-5 s, synthetic initial state from the dam-break generator.  A
+step is one pass of the hot path -- grid rebuilds plus SPHDEM_OneStep --
+over the resident state; `value` = fluid particles x K / wall seconds of K
+steps (inputs already in HBM).
+
+Also reported:
+  roofline      the pass with the largest GPU time in the timed region:
+                its algorithmic bytes per launch (each array it reads or
+                writes counted once, per the table below) over its average
+                launch time from HIP events recorded around every launch in
+                the timed region, against 8 TB/s.
+  cpu_baseline  the CPU oracle (C, OpenMP on this host's cores) timed on one
+                coupled step of the same workload.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch of each pass: (per fluid, per wall, per DEM).
+PASS_BYTES = {
+    "grid_build": (68.0, 0.0, 0.0),
+    "fluid_density": (56.0, 16.0, 0.0),
+    "wall_density": (16.0, 60.0, 0.0),
+    "fluid_force": (92.0, 40.0, 0.0),
+    "porosity": (28.0, 0.0, 32.0),
+    "interaction_dem": (52.0, 0.0, 120.0),
+    "fluid_final": (88.0, 0.0, 44.0),
+    "dem_csr": (0.0, 0.0, 40.0),
+    "dem_contact": (0.0, 0.0, 160.0),
+    "step_dem": (0.0, 0.0, 100.0),
+    "other": (0.0, 0.0, 0.0),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: workload's)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=("c0", "c1", "c2"))
+    ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for cpu_baseline")
+    ap.add_argument("--cpu-stride", type=int, default=None,
+                    help="oracle subject stride (default 8 for c2, else 1)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from warp_sph_gpt_amd import _lib
+    from warp_sph_gpt_amd.engine import CoupledEngine
+    from warp_sph_gpt_amd.workloads import WORKLOADS
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    wl = WORKLOADS[args.workload](device="cuda:0")
+    P, B, D, W = (s.to(dev) for s in wl.structs)
+    steps = args.steps if args.steps is not None else wl.steps
+    log(f"[bench] {wl.name}: {P.n} fluid, {B.n} wall, {D.n} DEM, {W.n} DEM-wall; {wl.note}; "
+        f"setup {time.time() - t0:.1f}s")
+    eng = CoupledEngine(wl.solv, P, B, D, W)
+    for _ in range(args.warmup):
+        eng.step()
+    torch.cuda.synchronize()
+    if args.graph:
+        eng.capture()
+    log(f"[bench] warmup {args.warmup} steps done")
+
+    torch.cuda.synchronize()
+    with _lib.PassTimer() as timer:
+        start = time.perf_counter()
+        eng.run(steps)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - start
+    eng.check()
+    diag = eng.diagnostics()
+    log(f"[bench] {steps} steps in {elapsed:.3f}s; per-pass ms: "
+        + ", ".join(f"{k}={v:.1f}" for k, v in timer.ms.items() if v > 0))
+
+    value = P.n * steps / elapsed
+    dom = max(timer.ms, key=lambda k: timer.ms[k])
+    count = max(timer.count[dom], 1)
+    avg_s = timer.ms[dom] / count * 1e-3
+    bf, bb, bd = PASS_BYTES[dom]
+    bytes_launch = bf * P.n + bb * B.n + bd * D.n
+    achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": dom, "avg_launch_ms": round(avg_s * 1e3, 4),
+                "bytes_per_launch": int(bytes_launch)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        stride = args.cpu_stride or (8 if wl.name == "c2" else 1)
+        cpu = cpu_baseline(wl, args.cpu_steps, stride)
+
+    line = {
+        "metric": "SPH particle-steps/s at 1.56M ptl; surrogate speedup at <=1% rho error",
+        "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name}: coupled WCSPH-DEM dam break", "fluid": P.n,
+                   "wall": B.n, "dem": D.n, "dem_wall": W.n, "dt": wl.solv.dt,
+                   "graph": bool(args.graph), "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
+        "diagnostics_fp64": diag,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(wl, n_steps: int, stride: int) -> dict:
+    """The C oracle (OpenMP) on the same workload.  With stride k > 1 the
+    fluid/wall neighbour loops visit every k-th subject only; the per-step
+    time is then estimated as grid builds + k x (the rest of the step)."""
+    from oracle import oracle as O
+
+    structs = [O.OStruct.from_struct(s) for s in wl.structs]
+    run = O.Run(wl.solv, *structs)
+    O.set_stride(stride)
+    t_grid = t_rest = 0.0
+    try:
+        for _ in range(n_steps):
+            t = time.perf_counter()
+            run.gs.build(run.P.pos, run.solv.support)
+            run.gd.build(run.D.pos, max(2.0 * run.scales[0], 2.0 * run.scales[2]))
+            t_grid += time.perf_counter() - t
+            t = time.perf_counter()
+            O.SPHDEM_OneStep(run.solv, run.P, run.B, run.D, run.W, run.gs, run.gb, run.gd,
+                             run.gw, run.step_index, run.scales)
+            run.step_index += 1
+            t_rest += time.perf_counter() - t
+    finally:
+        O.set_stride(1)
+    est = t_grid + stride * t_rest
+    nf = structs[0].n
+    log(f"[bench] cpu oracle: {n_steps} step(s): grids {t_grid:.2f}s, rest {t_rest:.2f}s at "
+        f"stride {stride}, estimated {est:.2f}s on {O.num_threads()} threads")
+    sample = f"{n_steps} coupled step(s) of {wl.name} ({nf} fluid) incl. grid builds"
+    if stride > 1:
+        sample += (f"; neighbour loops sampled on every {stride}th subject, "
+                   f"time = grids + {stride} x rest")
+    return {"value": round(nf * n_steps / est, 1), "unit": "particle-steps/s",
+            "cores": O.num_threads(), "kind": "port", "sample": sample}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -224,6 +224,26 @@ int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t 
                         const sphdem_grid_t *gw, int step, int stage,
                         const sphdem_work_t *w, void *stream);
 
+/* ---- per-pass timing (no reference counterpart) ------------------------ */
+/* When enabled, every pass records a pair of HIP events on its stream;
+ * sphdem_profile_read waits for them and adds the elapsed milliseconds and
+ * call counts per pass id (array index = id).  Not for use under graph
+ * capture. */
+#define SPHDEM_PASS_GRID 0          /* hash-grid build */
+#define SPHDEM_PASS_FLUID_DENSITY 1 /* Shepard + density + EOS, fluid */
+#define SPHDEM_PASS_WALL_DENSITY 2  /* density + EOS, wall */
+#define SPHDEM_PASS_FLUID_FORCE 3   /* acceleration (+ -grad p) */
+#define SPHDEM_PASS_DEM_CSR 4       /* contact counts + scan */
+#define SPHDEM_PASS_DEM_CONTACT 5   /* DEM-DEM / DEM-wall loads + CSR fill */
+#define SPHDEM_PASS_POROSITY 6      /* coupling-scale porosity */
+#define SPHDEM_PASS_INTERACTION_DEM 7
+#define SPHDEM_PASS_FLUID_FINAL 8   /* drag reaction + integration */
+#define SPHDEM_PASS_STEP_DEM 9
+#define SPHDEM_PASS_OTHER 10
+#define SPHDEM_NUM_PASSES 11
+int sphdem_profile_enable(int on);
+int sphdem_profile_read(double *ms, int *counts, int n);
+
 /* ---- FP64 diagnostics (no reference counterpart) ----------------------- */
 /* out[0] = total fluid mass, out[1] = fluid KE 1/2 sum m|v|^2,
  * out[2] = DEM KE incl. 1/2 I|w|^2, out[3] = DEM mass, out[4] = surge front

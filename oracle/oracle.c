@@ -6,6 +6,12 @@
 
 #define PI_D 3.14159265358979323846
 
+/* Subject stride of the fluid/wall neighbour loops: 1 computes every
+ * subject.  k > 1 computes every k-th subject only -- a bounded sample used
+ * to time the oracle on large workloads, never to produce results. */
+static int ora_stride = 1;
+void ora_set_stride(int k) { ora_stride = k > 0 ? k : 1; }
+
 /* FP32 pow: evaluated in double and rounded once to float (the correctly
  * rounded value up to a double-rounding tie).  Warp's float pow is an
  * implementation of the same function with a few ulp of error; this fixes

@@ -459,3 +459,8 @@ class Run:
         else:
             SPH_OneStep(s, self.P, self.B, self.gs, self.gb, self.step_index)
         self.step_index += 1
+
+
+def set_stride(k: int) -> None:
+    """Compute only every k-th fluid/wall subject (timing samples only)."""
+    lib().c.ora_set_stride(int(k))

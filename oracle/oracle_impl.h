@@ -197,7 +197,7 @@ void FN(ora_neighbour_counts)(const FN(ora_grid) *gs, const REAL *spos,
 void FN(ora_shepard_sph)(FN(ora_sph) *P, const FN(ora_bnd) *B, const FN(ora_grid) *gs,
                          const FN(ora_grid) *gb, REAL support, REAL h) {
     #pragma omp parallel for schedule(dynamic, 64)
-    for (int tid = 0; tid < P->n; ++tid) {
+    for (int tid = 0; tid < P->n; tid += ora_stride) {
         int i = gs->point_ids[tid];
         V3 ri = FN(ld3)(P->pos, i);
         REAL flt = 0;
@@ -222,7 +222,7 @@ void FN(ora_shepard_sph)(FN(ora_sph) *P, const FN(ora_bnd) *B, const FN(ora_grid
 void FN(ora_density_sph)(FN(ora_sph) *P, const FN(ora_bnd) *B, const FN(ora_grid) *gs,
                          const FN(ora_grid) *gb, REAL support, REAL h) {
     #pragma omp parallel for schedule(dynamic, 64)
-    for (int tid = 0; tid < P->n; ++tid) {
+    for (int tid = 0; tid < P->n; tid += ora_stride) {
         int i = gs->point_ids[tid];
         V3 ri = FN(ld3)(P->pos, i);
         REAL rho = 0;
@@ -247,7 +247,7 @@ void FN(ora_density_sph)(FN(ora_sph) *P, const FN(ora_bnd) *B, const FN(ora_grid
 void FN(ora_density_bnd)(const FN(ora_sph) *P, FN(ora_bnd) *B, const FN(ora_grid) *gs,
                          const FN(ora_grid) *gb, REAL support, REAL h) {
     #pragma omp parallel for schedule(dynamic, 64)
-    for (int tid = 0; tid < B->n; ++tid) {
+    for (int tid = 0; tid < B->n; tid += ora_stride) {
         int bi = gb->point_ids[tid];
         V3 rb = FN(ld3)(B->pos, bi);
         REAL rho = 0;
@@ -282,7 +282,7 @@ void FN(ora_force_sph)(FN(ora_sph) *P, const FN(ora_bnd) *B, const FN(ora_grid) 
                        const FN(ora_grid) *gb, REAL support, REAL h, REAL mu, REAL g) {
     const REAL r2min = (REAL)1.0e-12;
     #pragma omp parallel for schedule(dynamic, 64)
-    for (int tid = 0; tid < P->n; ++tid) {
+    for (int tid = 0; tid < P->n; tid += ora_stride) {
         int i = gs->point_ids[tid];
         V3 ri = FN(ld3)(P->pos, i), vi = FN(ld3)(P->vel, i);
         REAL rhoi = P->rho[i], presi = P->pres[i];
@@ -504,7 +504,7 @@ void FN(ora_prep_sphdem)(FN(ora_sph) *P, const FN(ora_bnd) *B, const FN(ora_dem)
                          REAL eps_min, REAL eps_max, int skip_fluid_sum) {
     const REAL r2min = (REAL)1.0e-12;
     #pragma omp parallel for schedule(dynamic, 32)
-    for (int tid = 0; tid < P->n; ++tid) {
+    for (int tid = 0; tid < P->n; tid += ora_stride) {
         int i = gs->point_ids[tid];
         V3 ri = FN(ld3)(P->pos, i);
         REAL flt = 0, solid = 0;
@@ -621,7 +621,7 @@ void FN(ora_interaction_dem)(const FN(ora_sph) *P, FN(ora_dem) *D, const FN(ora_
 void FN(ora_interaction_sph)(FN(ora_sph) *P, const FN(ora_dem) *D, const FN(ora_grid) *gs,
                              const FN(ora_grid) *gd, REAL support) {
     #pragma omp parallel for schedule(dynamic, 64)
-    for (int tid = 0; tid < P->n; ++tid) {
+    for (int tid = 0; tid < P->n; tid += ora_stride) {
         int i = gs->point_ids[tid];
         V3 ri = FN(ld3)(P->pos, i);
         V3 fd = FN(mk3)(0, 0, 0);

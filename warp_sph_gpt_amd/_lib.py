@@ -124,7 +124,31 @@ _SIGNATURES = {
                              c_int, c_int, POINTER(WorkStruct), c_void_p], c_int),
     "sphdem_reduce_f64": ([c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                            c_void_p, c_void_p, c_int, c_void_p, c_void_p], c_int),
+    "sphdem_profile_enable": ([c_int], c_int),
+    "sphdem_profile_read": ([c_void_p, c_void_p, c_int], c_int),
 }
+
+PASS_NAMES = ("grid_build", "fluid_density", "wall_density", "fluid_force", "dem_csr",
+              "dem_contact", "porosity", "interaction_dem", "fluid_final", "step_dem", "other")
+
+
+class PassTimer:
+    """Per-pass HIP-event timing of everything enqueued while active."""
+
+    def __enter__(self):
+        torch.cuda.synchronize()
+        call("sphdem_profile_enable", 1)
+        return self
+
+    def __exit__(self, *exc):
+        n = len(PASS_NAMES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        call("sphdem_profile_read", ctypes.addressof(ms), ctypes.addressof(cnt), n)
+        call("sphdem_profile_enable", 0)
+        self.ms = {PASS_NAMES[i]: ms[i] for i in range(n)}
+        self.count = {PASS_NAMES[i]: cnt[i] for i in range(n)}
+        return False
 
 _lib = None
 

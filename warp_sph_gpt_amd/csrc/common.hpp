@@ -29,6 +29,16 @@ int check_launch(const char *what);
 int scan_scratch(int n);
 int exclusive_scan(const int *in, int *out, int n, int *scratch, hipStream_t st);
 
+// Per-pass event timing (grid.hip); a no-op unless sphdem_profile_enable(1).
+int prof_begin(int pass, hipStream_t st);
+void prof_end(int slot, hipStream_t st);
+struct ProfScope {
+    int slot;
+    hipStream_t st;
+    ProfScope(int pass, hipStream_t s) : slot(prof_begin(pass, s)), st(s) {}
+    ~ProfScope() { prof_end(slot, st); }
+};
+
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int blocks_for(long n, int threads) { return (int)((n + threads - 1) / threads); }
 

@@ -440,6 +440,7 @@ template <typename T, bool WALL>
 static void launch_count(const sphdem_dem_t *D, const sphdem_grid_t *g, const void *nb_radius,
                          double rmax, hipStream_t st) {
     int *off = WALL ? D->contact_bnd_offset_new : D->contact_dem_offset_new;
+    ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
     hipLaunchKernelGGL((k_count_contacts<T, WALL>), dim3(blocks_for(D->n + 1, 64)), dim3(64), 0, st,
                        view<T>(D), view<T>(g), (const T *)nb_radius, (T)rmax, off);
 }
@@ -449,6 +450,7 @@ static void launch_contact(const sphdem_dem_t *D, const sphdem_grid_t *g, const 
                            const void *nb_vel, const void *nb_omega, double rmax, double grav,
                            double dt, int *flags, hipStream_t st) {
     if (D->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_DEM_CONTACT, st);
     hipLaunchKernelGGL((k_contact_force<T, WALL>), dim3(blocks_for(D->n, 64)), dim3(64), 0, st,
                        view<T>(D), view<T>(g), (const T *)nullptr, (const T *)nb_radius,
                        (const T *)nb_vel, (const T *)nb_omega, (T)rmax, (T)grav, (T)dt, flags);
@@ -458,6 +460,7 @@ template <typename T>
 static void launch_copy_csr(const sphdem_dem_t *D, bool wall, hipStream_t st) {
     const int cap = wall ? D->cap_bnd_new : D->cap_dem_new;
     const int span = (D->n + 1 > cap ? D->n + 1 : cap);
+    ProfScope prof(SPHDEM_PASS_DEM_CONTACT, st);
     if (wall)
         hipLaunchKernelGGL(k_copy_csr<T>, dim3(blocks_for(span, 256)), dim3(256), 0, st,
                            D->contact_bnd_offset_new, D->contact_bnd_id_new,
@@ -476,6 +479,7 @@ static void launch_porosity(const sphdem_params_t *p, const sphdem_sph_t *P,
                             const sphdem_grid_t *gd, const sphdem_work_t *w, double csup,
                             double hu, hipStream_t st) {
     if (gs->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_POROSITY, st);
     hipLaunchKernelGGL(k_porosity<T>, dim3(blocks_for(gs->n, kThreads)), dim3(kThreads), 0, st,
                        prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), view<T>(w),
                        (T)csup, (T)hu);
@@ -486,6 +490,7 @@ static void launch_interaction_dem(const sphdem_params_t *p, const sphdem_dem_t 
                                    const sphdem_grid_t *gs, const sphdem_grid_t *gd,
                                    const sphdem_work_t *w, hipStream_t st) {
     if (gd->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_INTERACTION_DEM, st);
     hipLaunchKernelGGL(k_interaction_dem<T>, dim3(gd->n), dim3(kThreads), 0, st, prm<T>(p),
                        view<T>(D), view<T>(gs), view<T>(gd), view<T>(w));
 }
@@ -495,6 +500,7 @@ static void launch_fluid_final(const sphdem_params_t *p, const sphdem_sph_t *P,
                                const sphdem_dem_t *D, const sphdem_grid_t *gs,
                                const sphdem_grid_t *gd, double csup, hipStream_t st) {
     if (gs->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_FLUID_FINAL, st);
     hipLaunchKernelGGL((k_fluid_final<T, INTER, STEP>), dim3(blocks_for(gs->n, kThreads)),
                        dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs),
                        view<T>(gd), (T)csup);
@@ -503,6 +509,7 @@ static void launch_fluid_final(const sphdem_params_t *p, const sphdem_sph_t *P,
 template <typename T>
 static void launch_step_dem(const sphdem_params_t *p, const sphdem_dem_t *D, hipStream_t st) {
     if (D->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_STEP_DEM, st);
     hipLaunchKernelGGL(k_step_dem<T>, dim3(blocks_for(D->n, 64)), dim3(64), 0, st, view<T>(D),
                        (T)p->dt);
 }
@@ -520,8 +527,11 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
         if (rc) return rc;
         launch_count<T, false>(D, gd, D->radius, p->radius_max, st);
-        rc = exclusive_scan(D->contact_dem_offset_new, D->contact_dem_offset_new, D->n + 1,
-                            w->scan_tmp, st);
+        {
+            ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
+            rc = exclusive_scan(D->contact_dem_offset_new, D->contact_dem_offset_new, D->n + 1,
+                                w->scan_tmp, st);
+        }
         if (rc) return rc;
     }
     if (all || stage == 1) {
@@ -529,8 +539,11 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
                                  w->flags, st);
         if (all) launch_copy_csr<T>(D, false, st);
         launch_count<T, true>(D, gw, W->radius, p->boundary_radius_max, st);
-        rc = exclusive_scan(D->contact_bnd_offset_new, D->contact_bnd_offset_new, D->n + 1,
-                            w->scan_tmp, st);
+        {
+            ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
+            rc = exclusive_scan(D->contact_bnd_offset_new, D->contact_bnd_offset_new, D->n + 1,
+                                w->scan_tmp, st);
+        }
         if (rc) return rc;
     }
     if (all || stage == 2) {

@@ -14,6 +14,7 @@
 #include "common.hpp"
 
 #include <cstdio>
+#include <vector>
 
 namespace sphdem {
 
@@ -30,6 +31,42 @@ int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(SPHDEM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
     return SPHDEM_OK;
+}
+
+// ---------------------------------------------------------------- timing
+namespace {
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // start/stop pairs
+    std::vector<int> pass;
+    size_t used = 0;
+};
+Prof g_prof;
+}  // namespace
+
+int prof_begin(int pass, hipStream_t st) {
+    if (!g_prof.on) return -1;
+    const size_t k = g_prof.used;
+    if (g_prof.pass.size() <= k) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return -1;
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return -1;
+        }
+        g_prof.ev.push_back(a);
+        g_prof.ev.push_back(b);
+        g_prof.pass.push_back(pass);
+    } else {
+        g_prof.pass[k] = pass;
+    }
+    if (hipEventRecord(g_prof.ev[2 * k], st) != hipSuccess) return -1;
+    g_prof.used = k + 1;
+    return (int)k;
+}
+
+void prof_end(int slot, hipStream_t st) {
+    if (slot >= 0) (void)hipEventRecord(g_prof.ev[2 * slot + 1], st);
 }
 
 // ---------------------------------------------------------------- scan
@@ -158,6 +195,7 @@ __global__ void k_grid_sorted(const T *pos, const T *w, const int *ids, int n, V
 template <typename T>
 static int grid_build(sphdem_grid_t *g, const void *points, int n, double radius, const void *w,
                       hipStream_t st) {
+    ProfScope prof(SPHDEM_PASS_GRID, st);
     const int ncell = g->dim_x * g->dim_y * g->dim_z;
     const T inv = (T)1.0 / (T)radius;
     g->cell_width = radius;
@@ -228,6 +266,33 @@ using namespace sphdem;
 extern "C" {
 
 int sphdem_version(void) { return 1; }
+
+int sphdem_profile_enable(int on) {
+    g_prof.on = on != 0;
+    if (g_prof.on) g_prof.used = 0;
+    return SPHDEM_OK;
+}
+
+int sphdem_profile_read(double *ms, int *counts, int n) {
+    SPHDEM_REQUIRE(n >= 0 && (n == 0 || (ms && counts)), "invalid profile buffers");
+    for (int i = 0; i < n; ++i) {
+        ms[i] = 0.0;
+        counts[i] = 0;
+    }
+    for (size_t k = 0; k < g_prof.used; ++k) {
+        if (hipEventSynchronize(g_prof.ev[2 * k + 1]) != hipSuccess)
+            return fail(SPHDEM_E_HIP, "profile event wait failed");
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, g_prof.ev[2 * k], g_prof.ev[2 * k + 1]) != hipSuccess)
+            return fail(SPHDEM_E_HIP, "profile elapsed time failed");
+        const int id = g_prof.pass[k];
+        if (id >= 0 && id < n) {
+            ms[id] += t;
+            counts[id] += 1;
+        }
+    }
+    return SPHDEM_OK;
+}
 
 const char *sphdem_last_error(void) { return g_err.c_str(); }
 

@@ -242,6 +242,7 @@ static void launch_fluid_density(const sphdem_params_t *p, const sphdem_sph_t *P
                                  const sphdem_bnd_t *B, const sphdem_grid_t *gs,
                                  const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
     if (gs->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_FLUID_DENSITY, st);
     WorkV<T> wv{};
     if (w) wv = view<T>(w);
     hipLaunchKernelGGL((k_fluid_density<T, SHEP, DENS, WORK>), dim3(blocks_for(gs->n, kThreads)),
@@ -254,6 +255,7 @@ static void launch_wall_density(const sphdem_params_t *p, const sphdem_sph_t *P,
                                 const sphdem_bnd_t *B, const sphdem_grid_t *gs,
                                 const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
     if (gb->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_WALL_DENSITY, st);
     WorkV<T> wv{};
     if (w) wv = view<T>(w);
     hipLaunchKernelGGL((k_wall_density<T, WORK>), dim3(blocks_for(gb->n, kThreads)), dim3(kThreads),
@@ -265,6 +267,7 @@ static void launch_fluid_force(const sphdem_params_t *p, const sphdem_sph_t *P,
                                const sphdem_bnd_t *B, const sphdem_grid_t *gs,
                                const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
     if (gs->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_FLUID_FORCE, st);
     hipLaunchKernelGGL((k_fluid_force<T, ACC, PGF, PUBLISH>), dim3(blocks_for(gs->n, kThreads)),
                        dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
                        view<T>(gb), view<T>(w));
@@ -273,6 +276,7 @@ static void launch_fluid_force(const sphdem_params_t *p, const sphdem_sph_t *P,
 template <typename T>
 void launch_gather(const sphdem_sph_t *P, const sphdem_bnd_t *B, const sphdem_grid_t *gs,
                    const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
+    ProfScope prof(SPHDEM_PASS_OTHER, st);
     if (P && gs && gs->n > 0)
         hipLaunchKernelGGL(k_gather_fluid<T>, dim3(blocks_for(gs->n, kThreads)), dim3(kThreads), 0,
                            st, view<T>(P), view<T>(gs), view<T>(w));
@@ -290,6 +294,7 @@ template void launch_gather<double>(const sphdem_sph_t *, const sphdem_bnd_t *,
 template <typename T>
 void launch_step_sph(const sphdem_params_t *p, const sphdem_sph_t *P, hipStream_t st) {
     if (P->n == 0) return;
+    ProfScope prof(SPHDEM_PASS_FLUID_FINAL, st);
     hipLaunchKernelGGL(k_step_sph<T>, dim3(blocks_for(P->n, kThreads)), dim3(kThreads), 0, st,
                        view<T>(P), (T)p->dt);
 }

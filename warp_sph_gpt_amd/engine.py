@@ -1,0 +1,148 @@
+"""Step loop without host round trips, for long runs and measurement.
+
+`CoupledEngine` runs what main.py:84-91 runs every step -- rebuild the
+moving grids from the current positions, then one SPH_OneStep /
+SPHDEM_OneStep -- with the same kernels as simulation.py, but:
+  * the contact CSR lives in capacity-managed buffers (csr_per_dem edges
+    per sphere and kind); the logical edge counts stay on the device, the
+    finished rows are copied into the read side instead of swapping, and a
+    device flag records overflow (checked by `check()`);
+  * every buffer address is fixed after construction, so one step can be
+    captured in a hipGraph and replayed (`run(..., graph=True)`): two
+    graphs, with and without the Shepard refresh.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .hashgrid import HashGrid
+from .input_reader import validate_dem_particles
+from .kernels import Workspace
+from .simulation import solver_params
+
+
+class CoupledEngine:
+    def __init__(self, solv, P_sph, P_bnd, P_dem=None, P_dem_bnd=None, csr_per_dem: int = 64,
+                 grid_slice: int | None = None):
+        self.solv = solv
+        self.P, self.B, self.D, self.W = P_sph, P_bnd, P_dem, P_dem_bnd
+        dev, dtype = P_sph.device, P_sph.dtype
+        self.device, self.dtype = dev, dtype
+        gsl = int(grid_slice or solv.grid_slice)
+        self.coupled = bool(solv.dem_enable and P_dem is not None)
+        self.gs = HashGrid(gsl, gsl, gsl, device=dev)
+        self.gb = HashGrid(gsl, gsl, gsl, device=dev)
+        self.gs.reserve(P_sph.n, dtype)
+        self.gb.build(P_bnd.pos, solv.support, weights=P_bnd.m)
+        self.ws = Workspace(dev, dtype)
+        nd = P_dem.n if self.coupled else 0
+        self.ws.reserve(P_sph.n, P_bnd.n, nd)
+        self.scales = None
+        if self.coupled:
+            self.scales = validate_dem_particles(solv, P_dem, P_dem_bnd)
+            self.gd = HashGrid(gsl, gsl, gsl, device=dev)
+            self.gw = HashGrid(gsl, gsl, gsl, device=dev)
+            self.gd.reserve(nd, dtype)
+            self.gw.build(P_dem_bnd.pos, max(2.0 * self.scales[1], solv.support))
+            self.r_dem = max(2.0 * self.scales[0], 2.0 * self.scales[2])
+            self._capacity_csr(max(int(csr_per_dem) * nd, 64))
+        self.params = solver_params(solv, dtype, self.scales)
+        self.step_index = 0
+        self._graphs = None
+
+    def _capacity_csr(self, cap: int) -> None:
+        D = self.D
+        for kind in ("dem", "bnd"):
+            ids_old = getattr(D, f"contact_{kind}_id_old")
+            tang_old = getattr(D, f"tang_{kind}_old")
+            E = int(ids_old.shape[0])
+            cap_k = max(cap, E)
+            new_ids = torch.zeros(cap_k, dtype=torch.int32, device=self.device)
+            new_tang = torch.zeros((cap_k, 3), dtype=self.dtype, device=self.device)
+            new_ids[:E] = ids_old
+            new_tang[:E] = tang_old
+            setattr(D, f"contact_{kind}_id_old", new_ids)
+            setattr(D, f"tang_{kind}_old", new_tang)
+            setattr(D, f"contact_{kind}_id_new", torch.zeros_like(new_ids))
+            setattr(D, f"tang_{kind}_new", torch.zeros_like(new_tang))
+        self.cap = cap
+
+    # ------------------------------------------------------------------ stepping
+    def _enqueue(self, step: int) -> None:
+        s = self.solv
+        self.gs.build(self.P.pos, s.support, weights=self.P.m)
+        st = _lib.stream_handle()
+        if self.coupled:
+            self.gd.build(self.D.pos, self.r_dem)
+            _lib.call("sphdem_coupled_step", self.params, _lib.sph_view(self.P),
+                      _lib.bnd_view(self.B), _lib.dem_view(self.D), _lib.dembnd_view(self.W),
+                      self.gs.struct(), self.gb.struct(), self.gd.struct(), self.gw.struct(),
+                      int(step), -1, self.ws.struct(), st)
+        else:
+            _lib.call("sphdem_sph_step", self.params, _lib.sph_view(self.P), _lib.bnd_view(self.B),
+                      self.gs.struct(), self.gb.struct(), int(step), 1, self.ws.struct(), st)
+
+    def shepard_due(self, step: int) -> bool:
+        k = self.solv.shepard_step
+        return step == 0 or (k > 0 and step % k == 0)
+
+    def step(self) -> None:
+        if self._graphs is not None:
+            self._graphs[self.shepard_due(self.step_index)].replay()
+        else:
+            self._enqueue(self.step_index)
+        self.step_index += 1
+
+    def capture(self) -> None:
+        """Capture one step with and one without the Shepard refresh.  The
+        steps they stand for must run on buffers of the same addresses;
+        capture runs the step once eagerly per graph first (warm-up)."""
+        side = torch.cuda.Stream(device=self.device)
+        graphs = {}
+        for shep in (True, False):
+            probe = 0 if shep else 1
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize()
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    self._enqueue(probe)
+            graphs[shep] = g
+        torch.cuda.synchronize()
+        self._graphs = graphs
+
+    def run(self, n_steps: int) -> None:
+        for _ in range(int(n_steps)):
+            self.step()
+
+    def check(self) -> None:
+        """Raise OverflowError if a contact row did not fit the capacity."""
+        if self.coupled and int(self.ws.flags[0].item()) & 1:
+            raise OverflowError(f"contact CSR exceeded {self.cap} edges per kind; "
+                                "construct the engine with a larger csr_per_dem")
+
+    def publish_csr(self) -> None:
+        """Trim the read-side CSR to exactly E edges, as SPHDEM_OneStep leaves it."""
+        if not self.coupled:
+            return
+        D = self.D
+        n = D.n
+        for kind in ("dem", "bnd"):
+            E = int(getattr(D, f"contact_{kind}_offset_old")[n].item())
+            setattr(D, f"contact_{kind}_id_old", getattr(D, f"contact_{kind}_id_old")[:E].clone())
+            setattr(D, f"tang_{kind}_old", getattr(D, f"tang_{kind}_old")[:E].clone())
+        self._graphs = None
+
+    def diagnostics(self) -> dict:
+        """FP64 totals: fluid mass and KE, DEM mass and KE (incl. spin), front."""
+        out = torch.zeros(5, dtype=torch.float64, device=self.device)
+        P, D = self.P, self.D
+        nd = D.n if self.coupled else 0
+        _lib.call("sphdem_reduce_f64", _lib.precision_of(self.dtype), _lib.ptr(P.pos),
+                  _lib.ptr(P.vel), _lib.ptr(P.m), P.n,
+                  _lib.ptr(D.vel) if nd else None, _lib.ptr(D.omega) if nd else None,
+                  _lib.ptr(D.m) if nd else None, _lib.ptr(D.inertia) if nd else None, nd,
+                  out.data_ptr(), _lib.stream_handle())
+        v = out.cpu().tolist()
+        return {"fluid_mass": v[0], "fluid_ke": v[1], "dem_ke": v[2], "dem_mass": v[3],
+                "front": v[4]}

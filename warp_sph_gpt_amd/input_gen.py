@@ -330,50 +330,69 @@ def check_dem_inside(pos: np.ndarray, radius, s: GenerationConfig) -> None:
 
 
 def rsa_spheres(n: int, radius, box_lo, box_hi, seed: int, attempts: int = 200_000,
-                grow_z: float = 0.0, max_grow: int = 200, r_max: float | None = None):
+                grow_z: float = 0.0, max_grow: int = 200, r_max: float | None = None,
+                batch: int = 4096):
     """Seeded random sequential addition of non-overlapping spheres.
 
-    Centres are drawn uniformly in [box_lo, box_hi]; a candidate is kept when
-    its centre distance to every kept sphere is at least the sum of radii.
-    `radius` is a scalar or a callable rng -> radius (then r_max bounds it).  If `attempts` draws
-    do not place n spheres and grow_z > 0, the box top is raised by grow_z
-    and drawing continues (at most max_grow times).  Returns (centres,
-    radii, box_hi_used) with float64 arrays in placement order.
+    Centres are drawn uniformly in [box_lo, box_hi] (numpy default_rng(seed),
+    three doubles per draw); a draw is kept when its distance to every kept
+    centre is at least the sum of the radii.  `radius` is a scalar or a
+    sampler (rng, size) -> radii bounded by r_max.  If `attempts` draws do
+    not place n spheres and grow_z > 0, the box top is raised by grow_z and
+    drawing continues (at most max_grow times).  Draws are tested in blocks
+    against the kept spheres through a bucket grid, then in draw order
+    against each other, which keeps exactly the sequential outcome.
+    Returns (centres, radii, box_hi_used) in placement order.
     """
     rng = np.random.default_rng(seed)
     lo = np.asarray(box_lo, np.float64)
     hi = np.asarray(box_hi, np.float64).copy()
+    sampler = callable(radius)
+    if sampler and r_max is None:
+        raise ValueError("r_max is required with a radius sampler")
+    rmax = float(r_max) if sampler else float(radius)
+    cell = 2.0 * rmax
+    top = hi[2] + (grow_z * max_grow if grow_z > 0 else 0.0)
+    dims = np.ceil((np.array([hi[0], hi[1], top]) - lo) / cell).astype(int) + 3
+    slots = 8  # centres at least `cell` apart: at most 8 per cubic bucket
+    table = np.full((*dims, slots), -1, np.int64)
+    fill = np.zeros(dims, np.int64)
     pos = np.zeros((n, 3))
     rad = np.zeros(n)
     k = 0
-    if callable(radius) and r_max is None:
-        raise ValueError("r_max is required with a radius sampler")
-    cell = 2.0 * (float(radius) if r_max is None else float(r_max))
-    buckets: dict = {}
+    offs = np.array([(i, j, l) for i in (-1, 0, 1) for j in (-1, 0, 1) for l in (-1, 0, 1)])
+
+    def bucket(c):
+        return np.floor((c - lo) / cell).astype(int) + 1
+
     for _ in range(max_grow + 1):
-        for _ in range(attempts):
-            if k == n:
-                break
-            r = float(radius(rng)) if callable(radius) else float(radius)
-            c = lo + rng.random(3) * (hi - lo)
-            key = tuple((c // cell).astype(int))
-            ok = True
-            for dx in (-1, 0, 1):
-                for dy in (-1, 0, 1):
-                    for dz in (-1, 0, 1):
-                        for j in buckets.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
-                            if np.sum((pos[j] - c) ** 2) < (rad[j] + r) ** 2:
-                                ok = False
-                                break
-                        if not ok:
-                            break
-                    if not ok:
-                        break
-                if not ok:
+        left = attempts
+        while left > 0 and k < n:
+            m = min(batch, left)
+            left -= m
+            c = lo + rng.random((m, 3)) * (hi - lo)
+            r = radius(rng, m) if sampler else np.full(m, rmax)
+            bc = bucket(c)
+            nb = bc[:, None, :] + offs[None]
+            nb = np.clip(nb, 0, dims - 1)
+            ids = table[nb[..., 0], nb[..., 1], nb[..., 2]].reshape(m, -1)
+            valid = ids >= 0
+            idx = np.where(valid, ids, 0)
+            d2 = np.sum((pos[idx] - c[:, None, :]) ** 2, axis=2)
+            clash = valid & (d2 < (rad[idx] + r[:, None]) ** 2)
+            ok = ~clash.any(axis=1)
+            start = k
+            for j in np.nonzero(ok)[0]:
+                if k == n:
                     break
-            if ok:
-                pos[k], rad[k] = c, r
-                buckets.setdefault(key, []).append(k)
+                if k > start:
+                    dd = np.sum((pos[start:k] - c[j]) ** 2, axis=1)
+                    if np.any(dd < (rad[start:k] + r[j]) ** 2):
+                        continue
+                pos[k], rad[k] = c[j], r[j]
+                b = tuple(bc[j])
+                table[b][fill[b]] = k
+                fill[b] += 1
                 k += 1
         if k == n or grow_z <= 0.0:
             break
