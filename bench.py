@@ -58,6 +58,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=("c0", "c1", "c2"))
     ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs")
+    ap.add_argument("--math", choices=("exact", "fast"), default="fast",
+                    help="W/dW arithmetic of the neighbour sums (parity-tested both ways)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for cpu_baseline")
     ap.add_argument("--cpu-stride", type=int, default=None,
                     help="oracle subject stride (default 8 for c2, else 1)")
@@ -81,7 +83,7 @@ def main(argv=None) -> int:
     steps = args.steps if args.steps is not None else wl.steps
     log(f"[bench] {wl.name}: {P.n} fluid, {B.n} wall, {D.n} DEM, {W.n} DEM-wall; {wl.note}; "
         f"setup {time.time() - t0:.1f}s")
-    eng = CoupledEngine(wl.solv, P, B, D, W)
+    eng = CoupledEngine(wl.solv, P, B, D, W, fast_math=args.math == "fast")
     for _ in range(args.warmup):
         eng.step()
     torch.cuda.synchronize()
@@ -125,7 +127,7 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"{wl.name}: coupled WCSPH-DEM dam break", "fluid": P.n,
                    "wall": B.n, "dem": D.n, "dem_wall": W.n, "dt": wl.solv.dt,
-                   "graph": bool(args.graph), "parallelism": "single GPU"},
+                   "graph": bool(args.graph), "math": args.math, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu,
         "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
         "diagnostics_fp64": diag,

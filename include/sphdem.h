@@ -46,6 +46,9 @@ typedef struct {
     double boundary_radius_max;  /* search_scales[1]: max DEM-wall radius */
     double h_max;                /* search_scales[2]: max DEM coupling h */
     double h_uniform;            /* search_scales[3]: common DEM h or 0 */
+    int fast_math;               /* 0: W, dW/dr with IEEE division and square
+                                    root, as the reference writes them;
+                                    1: hardware square root, 1/h multiplies */
 } sphdem_params_t;
 
 /* input/struct.py:3-21 SPHptl */
@@ -109,6 +112,10 @@ typedef struct {
     void *wl_vel;                   /* [nb*4] */
     int *scan_tmp;                  /* [sphdem_scan_scratch(n_dem + 1)] */
     int *flags;                     /* [4] bit 0: DEM CSR overflow */
+    int *fl_flag;                   /* [nf] near-DEM flags */
+    int *fl_list;                   /* [2*nf] porosity targets, flag offsets */
+    int *fl_count;                  /* [1] number of porosity targets */
+    int *fl_scan;                   /* [sphdem_scan_scratch(nf)] */
     int cap_f, cap_b;
 } sphdem_work_t;
 

@@ -190,8 +190,17 @@ def test_dem_contact_kernels_and_csr():
     S.compare(["pos", "vel", "omega", "acc"], kinds=(2,))
 
 
+@pytest.fixture(params=["exact", "fast"])
+def math_mode(request):
+    from warp_sph_gpt_amd import _lib
+    old = _lib.fast_math()
+    _lib.set_fast_math(request.param == "fast")
+    yield request.param
+    _lib.set_fast_math(old)
+
+
 @pytest.mark.parametrize("uniform_h", [True, False])
-def test_coupling_kernels(uniform_h):
+def test_coupling_kernels(uniform_h, math_mode):
     S = Scene()
     s = S.solv
     (P, B, D, W), (gP, gB, gD, gW) = S.o, S.g
@@ -214,6 +223,9 @@ def test_coupling_kernels(uniform_h):
                          s.dem_porosity_min, s.dem_porosity_max)
     S.compare(["porosity", "pgf"])
     assert P.porosity.min() < 0.99
+    if math_mode == "exact":
+        # the coupling-scale sums add the same terms in the same order
+        np.testing.assert_array_equal(np_(gP.porosity), P.porosity)
     O.Kernel_interaction_dem(P, D, S.og[0], S.og[2], s.mu, s.dem_porosity_min,
                              s.dem_porosity_max, s.dt)
     K.Kernel_interaction_dem(gP, gD, S.gg[0], S.gg[2], s.mu, s.dem_porosity_min,
@@ -268,7 +280,7 @@ def test_sph_only_steps_match_oracle():
     S.compare(["pres"], floor=1e-2 * stiff, tol=5 * TOL)
 
 
-def test_coupled_steps_match_oracle_and_csr_bit_exact():
+def test_coupled_steps_match_oracle_and_csr_bit_exact(math_mode):
     S = Scene()
     s = S.solv
     P, B, D, W = S.o

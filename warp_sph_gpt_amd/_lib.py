@@ -24,7 +24,8 @@ class Params(ctypes.Structure):
                 ("rho0", c_double), ("c0", c_double), ("gamma", c_double), ("mu", c_double),
                 ("g", c_double), ("dt", c_double), ("eps_min", c_double), ("eps_max", c_double),
                 ("shepard_step", c_int), ("radius_max", c_double),
-                ("boundary_radius_max", c_double), ("h_max", c_double), ("h_uniform", c_double)]
+                ("boundary_radius_max", c_double), ("h_max", c_double), ("h_uniform", c_double),
+                ("fast_math", c_int)]
 
 
 def _ptr_fields(names):
@@ -67,7 +68,8 @@ class GridStruct(ctypes.Structure):
 
 class WorkStruct(ctypes.Structure):
     _fields_ = _ptr_fields(("fl_rho", "fl_pres", "fl_V", "fl_vel", "fl_pgf", "wl_rho",
-                            "wl_pres", "wl_vel", "scan_tmp", "flags")) + [
+                            "wl_pres", "wl_vel", "scan_tmp", "flags", "fl_flag", "fl_list",
+                            "fl_count", "fl_scan")) + [
         ("cap_f", c_int), ("cap_b", c_int)]
 
 
@@ -268,9 +270,24 @@ def dembnd_view(W) -> DemBndView:
     return v
 
 
+_FAST_MATH = [False]
+
+
+def set_fast_math(flag: bool) -> None:
+    """Default arithmetic of the neighbour sums: False evaluates W and dW/dr
+    with IEEE division and square root exactly as the reference writes them;
+    True uses the hardware square root and multiplies by 1/h."""
+    _FAST_MATH[0] = bool(flag)
+
+
+def fast_math() -> bool:
+    return _FAST_MATH[0]
+
+
 def params(precision: int, **kw) -> Params:
     p = Params()
     p.precision = precision
+    p.fast_math = 1 if _FAST_MATH[0] else 0
     for name, value in kw.items():
         setattr(p, name, value)
     return p

@@ -27,6 +27,11 @@ void launch_pgf_only(const sphdem_params_t *p, const sphdem_sph_t *P, const sphd
 int check_sph_pair(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
                    const sphdem_grid_t *gs, const sphdem_grid_t *gb);
 int check_work(const sphdem_work_t *w, int nf, int nb);
+// porosity.hip: mark / compact / LDS-tiled coupling-scale sums (uniform h).
+template <typename T>
+int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
+                          const sphdem_grid_t *gs, const sphdem_grid_t *gd,
+                          const sphdem_work_t *w, double csup, double hu, hipStream_t st);
 
 // ---------------------------------------------------------------- contact law
 // kernel/KERNEL_DEM_force.py:25-74 DEM_contact: linear spring-dashpot with
@@ -485,6 +490,22 @@ static void launch_porosity(const sphdem_params_t *p, const sphdem_sph_t *P,
                        (T)csup, (T)hu);
 }
 
+// Uniform DEM h: the tiled pipeline of porosity.hip.  Heterogeneous h needs
+// a fluid sum per DEM neighbour and h_b (KERNEL_SPHDEM_interaction.py:90-95):
+// one thread per subject as above.
+template <typename T>
+static int porosity(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
+                    const sphdem_grid_t *gs, const sphdem_grid_t *gd, const sphdem_work_t *w,
+                    double csup, double hu, hipStream_t st) {
+    if (hu > 0.0) {
+        SPHDEM_REQUIRE(w->fl_flag && w->fl_list && w->fl_count && w->fl_scan,
+                       "porosity work buffers are NULL");
+        return launch_porosity_tiled<T>(p, P, D, gs, gd, w, csup, hu, st);
+    }
+    launch_porosity<T>(p, P, D, gs, gd, w, csup, hu, st);
+    return check_launch("porosity");
+}
+
 template <typename T>
 static void launch_interaction_dem(const sphdem_params_t *p, const sphdem_dem_t *D,
                                    const sphdem_grid_t *gs, const sphdem_grid_t *gd,
@@ -551,7 +572,8 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
                                 p->dt, w->flags, st);
         if (all) launch_copy_csr<T>(D, true, st);
         const double csup = 2.0 * p->h_max;
-        launch_porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st);
+        rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st);
+        if (rc) return rc;
         launch_interaction_dem<T>(p, D, gs, gd, w, st);
         launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st);
         launch_step_dem<T>(p, D, st);
@@ -658,7 +680,8 @@ int sphdem_prep_sphdem(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *
     SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
     hipStream_t st = as_stream(stream);
     DISPATCH(p->precision, (launch_pgf_only<T>(p, P, B, gs, gb, w, st),
-                            launch_porosity<T>(p, P, D, gs, gd, w, support, h, st)));
+                            rc = porosity<T>(p, P, D, gs, gd, w, support, h, st)));
+    if (rc) return rc;
     return check_launch("prep_sphdem");
 }
 

@@ -1,0 +1,328 @@
+// porosity.hip -- the coupling-scale porosity of Kernel_prep_sphdem
+// (kernel/KERNEL_SPHDEM_interaction.py:74-96, :111) for uniform DEM h.
+//
+// For each fluid subject i the reference sums V_j W(|r_ij|, h_dem) over all
+// fluid j within 2 h_dem (~2.7e5 particles at dx = 0.004), then divides each
+// nearby DEM's V_b W by it.  Only subjects with a DEM within 2 h_b need the
+// sum (elsewhere solid = 0 and the porosity is clamp(1)), so:
+//   1. mark:   near-DEM flag per fluid slot; far subjects get clamp(1) here.
+//   2. stream: one wave per 64 consecutive fluid slots holding a flagged
+//              subject.  The wave walks the buckets of the union of its
+//              subjects' balls in the reference's query order (z, then y,
+//              then x, then sorted order inside a bucket), one candidate at a
+//              time for all 64 lanes: candidate data come through wave-uniform
+//              (scalar) loads, and each lane adds the term only if the
+//              candidate lies within 2 h_dem of its own subject.  Every
+//              candidate that passes the reference's distance test lies in
+//              the union and inside the subject's own query box, so each sum
+//              receives exactly the reference's terms in the reference's
+//              order; buckets that cannot hold such a term are skipped.
+// Math: EXACT evaluates W with IEEE division and square root as written in
+// kernel/Kernel_KNL.py; FAST uses the hardware square root, fused
+// multiply-adds, a multiply by 1/h and the factored polynomial C u^4 (1+2q)
+// with C applied once per sum.
+#include "common.hpp"
+
+#include <climits>
+#include <cmath>
+#include <limits>
+
+namespace sphdem {
+
+template <typename T> __device__ __forceinline__ T fast_sqrt(T x);
+template <> __device__ __forceinline__ float fast_sqrt<float>(float x) {
+    return __builtin_amdgcn_sqrtf(x);
+}
+template <> __device__ __forceinline__ double fast_sqrt<double>(double x) { return ::sqrt(x); }
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform_f(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T> __device__ __forceinline__ T wave_min(T v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const T o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const T o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Conservative distance from coordinate p to the points of bucket
+// coordinate c along one axis (int(x * inv) == c, truncation toward zero:
+// c = 0 spans (-w, w)); w is slightly enlarged so rounding never culls.
+template <typename T>
+__device__ __forceinline__ T axis_gap(T lo_p, T hi_p, int c, T w) {
+    const T lo = (T)(c <= 0 ? c - 1 : c) * w;
+    const T hi = (T)(c >= 0 ? c + 1 : c) * w;
+    T d = lo - hi_p;
+    const T e = lo_p - hi;
+    d = d > e ? d : e;
+    return d > (T)0 ? d : (T)0;
+}
+
+// Slot ranges of virtual x cells [xa, xb] (xa + 2^20 >= 0, at most dx cells)
+// in a bucket row: two ranges when the row wraps around the folded grid.
+__device__ __forceinline__ int row_slots(const int *lo, const int *hi, int base, int dx, int xa,
+                                         int xb, int &a1, int &b1, int &a2, int &b2) {
+    a1 = b1 = a2 = b2 = 0;
+    if (xb < xa) return 0;
+    const int len = xb - xa + 1;
+    const int cx = fold(xa, dx);
+    const int first = min(len, dx - cx);
+    a1 = lo[base + cx];
+    b1 = hi[base + cx + first - 1];
+    if (len > first) {
+        a2 = lo[base];
+        b2 = hi[base + len - first - 1];
+        return 2;
+    }
+    return 1;
+}
+
+template <typename T, bool FAST>
+__device__ __forceinline__ void add_term(T &acc, const V4<T> &me, T qx, T qy, T qz, T V, T thr2,
+                                         T h, T inv_h) {
+    const T dx = me.x - qx, dy = me.y - qy, dz = me.z - qz;
+    if (FAST) {
+        const T td = fma(dz, dz, fma(dy, dy, dx * dx));
+        const T qq = fast_sqrt(td) * inv_h;
+        const T u = fma((T)-0.5, qq, (T)1.0);
+        T t = u * u;
+        t = t * t;
+        const T c = fma(V, t * fma((T)2.0, qq, (T)1.0), acc);
+        acc = td < thr2 ? c : acc;
+    } else {
+        const T td = dx * dx + dy * dy + dz * dz;
+        if (td < thr2) acc = acc + V * wendland_w(msqrt(td), h);
+    }
+}
+
+// Add the terms of sorted slots [a, b) for every lane of the wave.  The
+// wave loads 64 candidates at a time with one coalesced load per lane
+// (positions and m/rho packed as (x, y, z, V)), stages them in its own LDS
+// slot and every lane reads them back as broadcasts, in slot order; the
+// next 64 are loaded while these are processed.  Padding past b sits far
+// away and fails the distance test.
+template <typename T, bool FAST>
+__device__ __forceinline__ void stream_range(T &acc, const V4<T> &me, int a, int b,
+                                             const V4<T> *pos, const T *Vs, V4<T> *buf, int lane,
+                                             T thr2, T h, T inv_h) {
+    if (b <= a) return;
+    const T far = (T)1.0e30;
+    auto load = [&](int s) {
+        V4<T> c = {far, far, far, (T)0};
+        if (s < b) {
+            c = pos[s];
+            c.w = Vs[s];
+        }
+        return c;
+    };
+    V4<T> cur = load(a + lane);
+    for (int s0 = a; s0 < b; s0 += 64) {
+        buf[lane] = cur;
+        __builtin_amdgcn_wave_barrier();
+        const V4<T> nxt = load(s0 + 64 + lane);
+        const int cnt = uniform_i(min(64, b - s0));
+        if (cnt == 64) {
+#pragma unroll 16
+            for (int j = 0; j < 64; ++j) {
+                const V4<T> q = buf[j];
+                add_term<T, FAST>(acc, me, q.x, q.y, q.z, q.w, thr2, h, inv_h);
+            }
+        } else {
+#pragma unroll 4
+            for (int j = 0; j < cnt; ++j) {
+                const V4<T> q = buf[j];
+                add_term<T, FAST>(acc, me, q.x, q.y, q.z, q.w, thr2, h, inv_h);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        cur = nxt;
+    }
+}
+
+// 1. near-DEM flag; far subjects get their porosity now.
+template <typename T>
+__global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T> gd, T csup,
+                           int *flag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const V4<T> me = gs.s[k];
+    bool near = false;
+    query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const V4<T> q = gd.s[s];
+            const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+            if (dist < (T)2.0 * D.h[gd.ids[s]]) near = true;
+        }
+    });
+    flag[k] = near ? 1 : 0;
+    if (!near) P.porosity[gs.ids[k]] = clampr((T)1.0, p.eps_min, p.eps_max);
+}
+
+// Stream the union of the balls of the wave's lanes with `in` set; other
+// lanes compare against a zero threshold and add nothing.  Returns false
+// (having done nothing) if that union spans more buckets than the folded grid
+// holds along some axis.
+template <typename T, bool FAST>
+__device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &gs, const T *Vs,
+                             V4<T> *wbuf, int lane, T csup, T thr2, T h, T inv_h) {
+    const T big = std::numeric_limits<T>::max();
+    const T bxl = uniform_f(wave_min(in ? me.x : big)), bxh = uniform_f(wave_max(in ? me.x : -big));
+    const T byl = uniform_f(wave_min(in ? me.y : big)), byh = uniform_f(wave_max(in ? me.y : -big));
+    const T bzl = uniform_f(wave_min(in ? me.z : big)), bzh = uniform_f(wave_max(in ? me.z : -big));
+    const T inv = gs.inv, w = ((T)1.0 / inv) * (T)1.00001;
+    const T r2c = csup * csup * (T)1.0001;
+    // union of the lanes' query boxes: int() is monotone, so the box of the
+    // extreme positions bounds every lane's [int((p-r)/w), int((p+r)/w)]
+    const int X0 = uniform_i((int)((bxl - csup) * inv)), X1 = uniform_i((int)((bxh + csup) * inv));
+    const int Y0 = uniform_i((int)((byl - csup) * inv)), Y1 = uniform_i((int)((byh + csup) * inv));
+    const int Z0 = uniform_i((int)((bzl - csup) * inv)), Z1 = uniform_i((int)((bzh + csup) * inv));
+    if (!(X1 - X0 < gs.dx && Y1 - Y0 < gs.dy && Z1 - Z0 < gs.dz && X0 + kOrigin >= 0 &&
+          Y0 + kOrigin >= 0 && Z0 + kOrigin >= 0))
+        return false;
+    const T thr = in ? thr2 : (T)0;
+    const int dxy = gs.dx * gs.dy;
+    for (int z = Z0; z <= Z1; ++z) {
+        const T gz = axis_gap(bzl, bzh, z, w);
+        if (gz * gz > r2c) continue;
+        const int cz = fold(z, gs.dz);
+        for (int y = Y0; y <= Y1; ++y) {
+            const T gy = axis_gap(byl, byh, y, w);
+            const T dd = gy * gy + gz * gz;
+            if (dd > r2c) continue;
+            const T chord = msqrt(r2c - dd);
+            const int xa = uniform_i(max(X0, (int)((bxl - chord) * inv) - 1));
+            const int xb = uniform_i(min(X1, (int)((bxh + chord) * inv) + 1));
+            if (xb < xa) continue;
+            const int row = cz * dxy + fold(y, gs.dy) * gs.dx;
+            int a1, b1, a2, b2;
+            row_slots(gs.lo, gs.hi, row, gs.dx, xa, xb, a1, b1, a2, b2);
+            a1 = uniform_i(a1); b1 = uniform_i(b1);
+            a2 = uniform_i(a2); b2 = uniform_i(b2);
+            stream_range<T, FAST>(flt, me, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+            stream_range<T, FAST>(flt, me, a2, b2, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+        }
+    }
+    return true;
+}
+
+// 2. one wave per 64 consecutive fluid slots.  The flagged lanes are
+// streamed in groups that share a bucket row (y, z) -- a wave of sorted
+// slots can straddle two rows -- and, when a row group is still too wide
+// (folded buckets), in sub-groups of 16 buckets along x; a sub-group that
+// still does not fit (grids narrower than the query) walks its own query
+// per lane.
+template <typename T, bool FAST>
+__global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
+                                                    GridV<T> gd, const T *Vs, const int *flag,
+                                                    T csup, T hu, T thr2) {
+    __shared__ V4<T> stage[256];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    V4<T> *wbuf = stage + (threadIdx.x & ~63);
+    const bool active = k < gs.n && flag[k] != 0;
+    const unsigned long long live = __ballot(active);
+    if (live == 0ull) return;
+    const int lead = (int)(k & ~63) + __ffsll((long long)live) - 1;
+    const V4<T> me = gs.s[active ? k : lead];
+    const T h = hu, inv_h = (T)1.0 / hu, inv = gs.inv;
+    const int ky = (int)(me.y * inv), kz = (int)(me.z * inv), kx = ((int)(me.x * inv)) >> 4;
+    T flt = (T)0;
+    unsigned long long todo = live;
+    while (todo) {
+        const int ld = __ffsll((long long)todo) - 1;
+        const int ly = __shfl(ky, ld, 64), lz = __shfl(kz, ld, 64);
+        const unsigned long long grp = __ballot(active && ky == ly && kz == lz) & todo;
+        const bool in = (grp >> lane) & 1ull;
+        if (!stream_group<T, FAST>(flt, me, in, gs, Vs, wbuf, lane, csup, thr2, h, inv_h)) {
+            unsigned long long rest = grp;
+            while (rest) {
+                const int l2 = __ffsll((long long)rest) - 1;
+                const int lx = __shfl(kx, l2, 64);
+                const unsigned long long sub = __ballot(kx == lx) & rest;
+                const bool in2 = (sub >> lane) & 1ull;
+                if (!stream_group<T, FAST>(flt, me, in2, gs, Vs, wbuf, lane, csup, thr2, h,
+                                           inv_h) && in2) {
+                    query_runs(gs, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+                        for (int s = lo; s < hi; ++s) {
+                            const V4<T> q = gs.s[s];
+                            add_term<T, FAST>(flt, me, q.x, q.y, q.z, Vs[s], thr2, h, inv_h);
+                        }
+                    });
+                }
+                rest &= ~sub;
+            }
+        }
+        todo &= ~grp;
+    }
+    if (!active) return;
+    if (FAST) flt = flt * ((T)21.0 / ((T)16.0 * (T)SPHDEM_PI * hu * hu * hu));
+    T solid = (T)0;
+    query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const int b = gd.ids[s];
+            const V4<T> q = gd.s[s];
+            const T hb = D.h[b];
+            const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+            if (dist < (T)2.0 * hb)
+                solid = solid + D.volume[b] * wendland_w(dist, hb) / (flt + (T)1.0e-20);
+        }
+    });
+    P.porosity[gs.ids[k]] = clampr((T)1.0 - solid, p.eps_min, p.eps_max);
+}
+
+// Smallest x with sqrt(x) >= s (IEEE, round to nearest): sqrt(td) < s is
+// then exactly td < thr2.
+template <typename T> T sqrt_threshold(T s) {
+    const T big = std::numeric_limits<T>::max();
+    T x = s * s;
+    while (x > (T)0 && std::sqrt(std::nextafter(x, (T)0)) >= s) x = std::nextafter(x, (T)0);
+    while (std::sqrt(x) < s) x = std::nextafter(x, big);
+    return x;
+}
+
+template <typename T>
+int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
+                          const sphdem_grid_t *gs, const sphdem_grid_t *gd,
+                          const sphdem_work_t *w, double csup, double hu, hipStream_t st) {
+    const int n = gs->n;
+    if (n == 0) return SPHDEM_OK;
+    ProfScope prof(SPHDEM_PASS_POROSITY, st);
+    const T cs = (T)csup;
+    hipLaunchKernelGGL(k_por_mark<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st, prm<T>(p),
+                       view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), cs, w->fl_flag);
+    const T thr2 = sqrt_threshold<T>(cs);
+    if (p->fast_math)
+        hipLaunchKernelGGL((k_por_stream<T, true>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
+                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2);
+    else
+        hipLaunchKernelGGL((k_por_stream<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
+                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2);
+    return check_launch("porosity");
+}
+template int launch_porosity_tiled<float>(const sphdem_params_t *, const sphdem_sph_t *,
+                                          const sphdem_dem_t *, const sphdem_grid_t *,
+                                          const sphdem_grid_t *, const sphdem_work_t *, double,
+                                          double, hipStream_t);
+template int launch_porosity_tiled<double>(const sphdem_params_t *, const sphdem_sph_t *,
+                                           const sphdem_dem_t *, const sphdem_grid_t *,
+                                           const sphdem_grid_t *, const sphdem_work_t *, double,
+                                           double, hipStream_t);
+
+}  // namespace sphdem

@@ -24,7 +24,7 @@ from .simulation import solver_params
 
 class CoupledEngine:
     def __init__(self, solv, P_sph, P_bnd, P_dem=None, P_dem_bnd=None, csr_per_dem: int = 64,
-                 grid_slice: int | None = None):
+                 grid_slice: int | None = None, fast_math: bool | None = None):
         self.solv = solv
         self.P, self.B, self.D, self.W = P_sph, P_bnd, P_dem, P_dem_bnd
         dev, dtype = P_sph.device, P_sph.dtype
@@ -48,6 +48,8 @@ class CoupledEngine:
             self.r_dem = max(2.0 * self.scales[0], 2.0 * self.scales[2])
             self._capacity_csr(max(int(csr_per_dem) * nd, 64))
         self.params = solver_params(solv, dtype, self.scales)
+        if fast_math is not None:
+            self.params.fast_math = 1 if fast_math else 0
         self.step_index = 0
         self._graphs = None
 

@@ -39,6 +39,11 @@ class Workspace:
             self.fl_V = torch.zeros(c, dtype=t, device=d)
             self.fl_vel = torch.zeros((c, 4), dtype=t, device=d)
             self.fl_pgf = torch.zeros((c, 4), dtype=t, device=d)
+            self.fl_flag = torch.zeros(c, dtype=torch.int32, device=d)
+            self.fl_list = torch.zeros(2 * c, dtype=torch.int32, device=d)
+            self.fl_count = torch.zeros(4, dtype=torch.int32, device=d)
+            self.fl_scan = torch.zeros(_lib.load().sphdem_scan_scratch(c), dtype=torch.int32,
+                                       device=d)
             self.cap_f = c
         if nb > self.cap_b:
             c = max(nb, 1)
@@ -54,7 +59,7 @@ class Workspace:
     def struct(self) -> _lib.WorkStruct:
         w = _lib.WorkStruct()
         for name in ("fl_rho", "fl_pres", "fl_V", "fl_vel", "fl_pgf", "wl_rho", "wl_pres",
-                     "wl_vel", "scan_tmp", "flags"):
+                     "wl_vel", "scan_tmp", "flags", "fl_flag", "fl_list", "fl_count", "fl_scan"):
             setattr(w, name, getattr(self, name).data_ptr())
         w.cap_f, w.cap_b = self.cap_f, self.cap_b
         return w
