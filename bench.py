@@ -64,7 +64,23 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-stride", type=int, default=None,
                     help="oracle subject stride (default 8 for c2, else 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                       "profiles", "porosity_c2_traffic.json"),
+                    help="JSON file with measured HBM bytes per launch of the dominant pass "
+                         "(from rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by a copy of known size)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only copy a buffer of this many MiB 5 times (counter calibration)")
     args = ap.parse_args(argv)
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n * 4}))
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -109,8 +125,14 @@ def main(argv=None) -> int:
     bf, bb, bd = PASS_BYTES[dom]
     bytes_launch = bf * P.n + bb * B.n + bd * D.n
     achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            t = json.load(f)
+        if t.get("kernel") == dom and t.get("workload") == wl.name:
+            traffic = int(t["bytes_per_launch"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                 "kernel": dom, "avg_launch_ms": round(avg_s * 1e3, 4),
                 "bytes_per_launch": int(bytes_launch)}
 
