@@ -1,0 +1,53 @@
+"""The C ABI: libsphdem.so loads and exports every function of include/sphdem.h
+(no device work; runs without a GPU), and the ctypes structures match the
+header's field lists."""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from warp_sph_gpt_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = (ROOT / "include" / "sphdem.h").read_text()
+
+
+def declared_functions():
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(sphdem_\w+)\s*\(", HEADER, re.M)))
+
+
+def struct_fields(name):
+    end = re.search(r"\}\s*" + name + r"\s*;", HEADER).start()
+    start = HEADER.rindex("typedef struct {", 0, end) + len("typedef struct {")
+    body = re.sub(r"/\*.*?\*/", "", HEADER[start:end], flags=re.S)
+    out = []
+    for decl in body.split(";"):
+        words = [w for w in re.split(r"[\s,*]+", decl) if w]
+        out.extend(w for w in words if w not in ("int", "double", "void", "const"))
+    return out
+
+
+def test_library_exports_every_declared_function():
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libsphdem.so is not built (run python -m warp_sph_gpt_amd.build)")
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.exported_symbols()), set(names) ^ set(_lib.exported_symbols())
+    assert lib.sphdem_version() == 1
+    assert lib.sphdem_scan_scratch(4096) >= 2
+
+
+@pytest.mark.parametrize("cname,pyname", [
+    ("sphdem_params_t", "Params"), ("sphdem_sph_t", "SphView"), ("sphdem_bnd_t", "BndView"),
+    ("sphdem_dem_t", "DemView"), ("sphdem_dembnd_t", "DemBndView"),
+    ("sphdem_grid_t", "GridStruct"), ("sphdem_work_t", "WorkStruct")])
+def test_ctypes_structures_follow_the_header(cname, pyname):
+    fields = struct_fields(cname)
+    got = [f[0] for f in getattr(_lib, pyname)._fields_]
+    assert got == fields, (cname, fields, got)
