@@ -56,8 +56,12 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: workload's)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=("c0", "c1", "c2"))
-    ap.add_argument("--graph", action="store_true", help="replay captured hipGraphs")
+    ap.add_argument("--workload", default="c2", choices=("c0", "c1", "c2", "c3"))
+    ap.add_argument("--graph", action="store_true",
+                    help="replay captured hipGraphs (always on for the c3 ensemble)")
+    ap.add_argument("--instances", type=int, default=64, help="c3: number of instances")
+    ap.add_argument("--tune-steps", type=int, default=2,
+                    help="c3: steps per autotuner candidate (0: library default shapes)")
     ap.add_argument("--math", choices=("exact", "fast"), default="fast",
                     help="W/dW arithmetic of the neighbour sums (parity-tested both ways)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for cpu_baseline")
@@ -91,6 +95,9 @@ def main(argv=None) -> int:
     from warp_sph_gpt_amd import _lib
     from warp_sph_gpt_amd.engine import CoupledEngine
     from warp_sph_gpt_amd.workloads import WORKLOADS
+
+    if args.workload == "c3":
+        return bench_ensemble(args)
 
     dev = torch.device("cuda:0")
     t0 = time.time()
@@ -153,6 +160,72 @@ def main(argv=None) -> int:
         "roofline": roofline, "cpu_baseline": cpu,
         "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
         "diagnostics_fp64": diag,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def roofline_of(timer, steps_counted, nf, nb, nd, traffic=None) -> dict:
+    dom = max(timer.ms, key=lambda k: timer.ms[k])
+    count = max(timer.count[dom], 1)
+    avg_s = timer.ms[dom] / count * 1e-3
+    bf, bb, bd = PASS_BYTES[dom]
+    bytes_launch = bf * nf + bb * nb + bd * nd
+    achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+            "kernel": dom, "avg_launch_ms": round(avg_s * 1e3, 4),
+            "bytes_per_launch": int(bytes_launch)}
+
+
+def bench_ensemble(args) -> int:
+    """C3: the batched ensemble, autotuned launch shapes, one hipGraph replay
+    per step for all instances.  value = fluid particles of all instances x
+    steps / wall seconds."""
+    from warp_sph_gpt_amd import _lib, autotune
+    from warp_sph_gpt_amd.ensemble import EnsembleEngine
+    from warp_sph_gpt_amd.workloads import ENSEMBLES
+
+    t0 = time.time()
+    wl = ENSEMBLES["c3"](device="cuda:0", n_inst=args.instances)
+    ens = EnsembleEngine(wl.solv, wl.scenes, device="cuda:0", fast_math=args.math == "fast")
+    nf, nb = ens.P.n, ens.B.n
+    nd = ens.D.n if ens.D is not None else 0
+    steps = args.steps if args.steps is not None else wl.steps
+    log(f"[bench] c3: {wl.note}; {nb} wall, {ens.W.n if ens.W is not None else 0} DEM-wall; "
+        f"setup {time.time() - t0:.1f}s")
+    tuned = None
+    if args.tune_steps > 0:
+        tuned = autotune.autotune(ens.step, steps=args.tune_steps, log=log)["tuning"]
+    for _ in range(args.warmup):
+        ens.step()
+    torch.cuda.synchronize()
+    with _lib.PassTimer() as timer:          # eager per-pass breakdown (2 steps)
+        for _ in range(2):
+            ens.step()
+        torch.cuda.synchronize()
+    ens.capture()
+    ens.step()
+    torch.cuda.synchronize()
+    log(f"[bench] autotuned, warmed up and captured at step {ens.step_index}")
+    start = time.perf_counter()
+    ens.run(steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    ens.check()
+    log(f"[bench] {steps} graph steps in {elapsed:.3f}s")
+    line = {
+        "metric": "SPH particle-steps/s, batched ensemble (all instances)",
+        "value": round(nf * steps / elapsed, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(elapsed / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"c3: {wl.note}", "instances": ens.n, "fluid": nf, "wall": nb,
+                   "dem": nd, "dt": wl.solv.dt, "graph": True, "math": args.math,
+                   "tuning": tuned, "parallelism": "single GPU"},
+        "roofline": roofline_of(timer, 2, nf, nb, nd),
+        "cpu_baseline": None,
+        "passes_ms_per_step_eager": {k: round(v / 2, 4) for k, v in timer.ms.items() if v > 0},
     }
     print(json.dumps(line), flush=True)
     return 0

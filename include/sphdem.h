@@ -101,6 +101,15 @@ typedef struct {
     int *cell_hi;         /* [ncell] one past the last slot of each bucket */
     void *sorted;         /* [n*4] (x, y, z, w) in sorted order, w from `w` */
     int *scan_tmp;        /* [sphdem_scan_scratch(ncell)] */
+    /* Batched instances (no reference counterpart): independent scenes share
+     * one grid, each with its own dim_x*dim_y*dim_z buckets, so ncell in the
+     * sizes above is dim_x*dim_y*dim_z*max(n_inst, 1).  inst[i] is the
+     * instance of point i and must be
+     * nondecreasing, so instance k owns one contiguous range of points, both
+     * in input and in sorted order.  A query by a point of instance k sees
+     * instance k's buckets only.  n_inst <= 1 and inst = NULL: one scene. */
+    int n_inst;
+    const int *inst;      /* [n] or NULL */
 } sphdem_grid_t;
 
 /* Per-step working arrays of the fused step, in grid-sorted order.
@@ -250,6 +259,25 @@ int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t 
 #define SPHDEM_NUM_PASSES 11
 int sphdem_profile_enable(int on);
 int sphdem_profile_read(double *ms, int *counts, int n);
+
+/* ---- launch shapes (no reference counterpart) -------------------------- */
+/* Process-wide launch shapes of the per-subject passes, set by the
+ * autotuner.  They change which lanes and workgroups evaluate which
+ * subjects, never the terms of a subject's sums or their order, so results
+ * are the same bits under every valid setting.  Honoured by: GRID,
+ * FLUID_DENSITY, WALL_DENSITY, FLUID_FORCE, POROSITY and FLUID_FINAL.
+ *   block[pass]      threads per workgroup: 64, 128 or 256 (default 256)
+ *   xcd_remap[pass]  1: hand consecutive subject tiles to one XCD, so that
+ *                    neighbouring tiles share that XCD's L2 (default 0)
+ *   por_x_tile       log2 of the buckets along x that one porosity stream
+ *                    sub-group may span: 2..6 (default 4) */
+typedef struct {
+    int block[SPHDEM_NUM_PASSES];
+    int xcd_remap[SPHDEM_NUM_PASSES];
+    int por_x_tile;
+} sphdem_tuning_t;
+int sphdem_tuning_get(sphdem_tuning_t *t);
+int sphdem_tuning_set(const sphdem_tuning_t *t);
 
 /* ---- FP64 diagnostics (no reference counterpart) ----------------------- */
 /* out[0] = total fluid mass, out[1] = fluid KE 1/2 sum m|v|^2,

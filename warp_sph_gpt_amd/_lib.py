@@ -63,7 +63,8 @@ class GridStruct(ctypes.Structure):
     _fields_ = [("precision", c_int), ("dim_x", c_int), ("dim_y", c_int), ("dim_z", c_int),
                 ("cell_width", c_double), ("n", c_int), ("point_cells", c_void_p),
                 ("point_ids", c_void_p), ("rank", c_void_p), ("cell_lo", c_void_p),
-                ("cell_hi", c_void_p), ("sorted", c_void_p), ("scan_tmp", c_void_p)]
+                ("cell_hi", c_void_p), ("sorted", c_void_p), ("scan_tmp", c_void_p),
+                ("n_inst", c_int), ("inst", c_void_p)]
 
 
 class WorkStruct(ctypes.Structure):
@@ -73,11 +74,21 @@ class WorkStruct(ctypes.Structure):
         ("cap_f", c_int), ("cap_b", c_int)]
 
 
+NUM_PASSES = 11
+
+
+class TuningStruct(ctypes.Structure):
+    _fields_ = [("block", c_int * NUM_PASSES), ("xcd_remap", c_int * NUM_PASSES),
+                ("por_x_tile", c_int)]
+
+
 _SIGNATURES = {
     "sphdem_version": ([], c_int),
     "sphdem_last_error": ([], ctypes.c_char_p),
     "sphdem_device_count": ([], c_int),
     "sphdem_scan_scratch": ([c_int], c_int),
+    "sphdem_tuning_get": ([POINTER(TuningStruct)], c_int),
+    "sphdem_tuning_set": ([POINTER(TuningStruct)], c_int),
     "sphdem_exclusive_scan": ([c_void_p, c_int, c_void_p, c_void_p], c_int),
     "sphdem_grid_build": ([POINTER(GridStruct), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
     "sphdem_grid_set_w": ([POINTER(GridStruct), c_void_p, c_void_p], c_int),
@@ -291,3 +302,28 @@ def params(precision: int, **kw) -> Params:
     for name, value in kw.items():
         setattr(p, name, value)
     return p
+
+
+# ---------------------------------------------------------------- launch shapes
+TUNABLE_PASSES = ("grid_build", "fluid_density", "wall_density", "fluid_force", "porosity",
+                  "fluid_final")
+
+
+def get_tuning() -> dict:
+    """The library's current launch shapes, by pass name."""
+    t = TuningStruct()
+    call("sphdem_tuning_get", t)
+    return {"block": {n: int(t.block[i]) for i, n in enumerate(PASS_NAMES)},
+            "xcd_remap": {n: int(t.xcd_remap[i]) for i, n in enumerate(PASS_NAMES)},
+            "por_x_tile": int(t.por_x_tile)}
+
+
+def set_tuning(cfg: dict) -> None:
+    """Set launch shapes; keys missing from `cfg` keep their current value."""
+    cur = get_tuning()
+    t = TuningStruct()
+    for i, n in enumerate(PASS_NAMES):
+        t.block[i] = int(cfg.get("block", {}).get(n, cur["block"][n]))
+        t.xcd_remap[i] = int(cfg.get("xcd_remap", {}).get(n, cur["xcd_remap"][n]))
+    t.por_x_tile = int(cfg.get("por_x_tile", cur["por_x_tile"]))
+    call("sphdem_tuning_set", t)

@@ -42,6 +42,30 @@ struct ProfScope {
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int blocks_for(long n, int threads) { return (int)((n + threads - 1) / threads); }
 
+// Buckets of a grid over all its instances.
+inline int grid_cells(const sphdem_grid_t *g) {
+    return g->dim_x * g->dim_y * g->dim_z * (g->n_inst > 1 ? g->n_inst : 1);
+}
+
+// Launch shapes set by sphdem_tuning_set (grid.hip).
+const sphdem_tuning_t &tuning();
+inline int tuned_block(int pass) { return tuning().block[pass]; }
+inline int tuned_remap(int pass) { return tuning().xcd_remap[pass]; }
+
+// Workgroup -> subject tile.  The dispatcher hands workgroup b to XCD b % 8;
+// with `remap` set, the workgroups of one XCD take one contiguous run of
+// tiles, so tiles of neighbouring (sorted, hence spatially close) subjects
+// share an L2.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int tile_id(int remap) {
+    const int b = blockIdx.x;
+    if (!remap) return b;
+    const int G = gridDim.x, per = G >> 3, rem = G & 7, x = b & 7, i = b >> 3;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+__device__ __forceinline__ int subject_id(int remap) {
+    return tile_id(remap) * blockDim.x + threadIdx.x;
+}
+
 // ---------------------------------------------------------------- vectors
 // Overloads that keep float math in float on both compilation passes.
 __device__ __forceinline__ float msqrt(float x) { return sqrtf(x); }
@@ -143,12 +167,24 @@ template <typename T> struct WallV {
     T *pos, *vel, *omega, *radius, *rho, *m, *volume, *inertia, *K, *eta, *mu;
     int n;
 };
+// A grid as the kernels see it.  `off` is the first bucket of the instance
+// being queried (instance * cells); set it with at_inst() before a query.
 template <typename T> struct GridV {
     int dx, dy, dz, n;
     T inv;
     const int *ids, *lo, *hi;
     const V4<T> *s;
+    const int *inst;  // instance of each point (input = sorted layout), or null
+    int cells;        // buckets per instance
+    int off;
 };
+template <typename T> __device__ __forceinline__ int inst_of(const GridV<T> &g, int i) {
+    return g.inst ? g.inst[i] : 0;
+}
+template <typename T> __device__ __forceinline__ GridV<T> at_inst(GridV<T> g, int k) {
+    g.off = k * g.cells;
+    return g;
+}
 template <typename T> struct WorkV {
     T *fl_rho, *fl_pres, *fl_V;
     V4<T> *fl_vel, *fl_pgf;
@@ -184,7 +220,8 @@ template <typename T> inline GridV<T> view(const sphdem_grid_t *g) {
     // Warp stores 1/cell_width rounded to the grid's float type.
     T inv = (T)1.0 / (T)g->cell_width;
     return {g->dim_x, g->dim_y, g->dim_z, g->n, inv, g->point_ids, g->cell_lo, g->cell_hi,
-            (const V4<T> *)g->sorted};
+            (const V4<T> *)g->sorted, g->n_inst > 1 ? g->inst : nullptr,
+            g->dim_x * g->dim_y * g->dim_z, 0};
 }
 template <typename T> inline WorkV<T> view(const sphdem_work_t *w) {
     return {(T *)w->fl_rho, (T *)w->fl_pres, (T *)w->fl_V, (V4<T> *)w->fl_vel,
@@ -215,6 +252,7 @@ __device__ __forceinline__ void query_runs(const GridV<T> &g, T px, T py, T pz, 
     const int z1 = min((int)((pz + r) * inv), z0 + g.dz - 1);
     if (g.n == 0) return;
     const int dxy = g.dx * g.dy;
+    const int *lo = g.lo + g.off, *hi = g.hi + g.off;
     const int len = x1 - x0 + 1;
     for (int z = z0; z <= z1; ++z) {
         const int cz = fold(z, g.dz);
@@ -223,12 +261,12 @@ __device__ __forceinline__ void query_runs(const GridV<T> &g, T px, T py, T pz, 
             if (x0 + kOrigin >= 0) {
                 const int cx = (x0 + kOrigin) % g.dx;
                 const int first = min(len, g.dx - cx);
-                if (first > 0) fn(g.lo[base + cx], g.hi[base + cx + first - 1]);
-                if (len > first) fn(g.lo[base], g.hi[base + len - first - 1]);
+                if (first > 0) fn(lo[base + cx], hi[base + cx + first - 1]);
+                if (len > first) fn(lo[base], hi[base + len - first - 1]);
             } else {
                 for (int x = x0; x <= x1; ++x) {
                     const int c = base + fold(x, g.dx);
-                    fn(g.lo[c], g.hi[c]);
+                    fn(lo[c], hi[c]);
                 }
             }
         }

@@ -61,10 +61,12 @@ __device__ __forceinline__ void dem_contact(V3<T> normal, T overlap, T ra, T rb,
 // KERNEL_DEM_force.py:9-22 and KERNEL_DEM_BC.py:10-24: overlap counts per row.
 // WALL selects the fixed-sphere set (no self exclusion).
 template <typename T, bool WALL>
-__global__ void k_count_contacts(DemV<T> D, GridV<T> g, const T *nb_radius, T rmax, int *off) {
+__global__ void k_count_contacts(DemV<T> D, GridV<T> g, const int *dinst, const T *nb_radius,
+                                 T rmax, int *off) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a == 0) off[D.n] = 0;
     if (a >= D.n) return;
+    if (dinst) g = at_inst(g, dinst[a]);
     const V3<T> ra = ld3(D.pos, a);
     const T Ra = D.radius[a];
     int count = 0;
@@ -84,11 +86,12 @@ __global__ void k_count_contacts(DemV<T> D, GridV<T> g, const T *nb_radius, T rm
 // (WALL = true): loads plus the new CSR row; history looked up by stable id
 // in the old row (last match wins).  Rows past `cap` set flags bit 0.
 template <typename T, bool WALL>
-__global__ void k_contact_force(DemV<T> D, GridV<T> g, const T *nb_pos_unused, const T *nb_radius,
+__global__ void k_contact_force(DemV<T> D, GridV<T> g, const int *dinst, const T *nb_radius,
                                 const T *nb_vel, const T *nb_omega, T rmax, T grav, T dt,
                                 int *flags) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= D.n) return;
+    if (dinst) g = at_inst(g, dinst[a]);
     const int *off_old = WALL ? D.bnd_off_old : D.dem_off_old;
     const int *id_old = WALL ? D.bnd_id_old : D.dem_id_old;
     const T *tang_old = WALL ? D.bnd_tang_old : D.dem_tang_old;
@@ -211,6 +214,9 @@ __global__ __launch_bounds__(kThreads) void k_porosity(Prm<T> p, SphV<T> P, DemV
     if (k >= gs.n) return;
     const V4<T> me = gs.s[k];
     const int i = gs.ids[k];
+    const int ik = inst_of(gs, k);
+    gs = at_inst(gs, ik);
+    gd = at_inst(gd, ik);
     bool near = false;
     query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
         for (int s = lo; s < hi; ++s) {
@@ -267,6 +273,9 @@ __global__ __launch_bounds__(kThreads) void k_interaction_dem(Prm<T> p, DemV<T> 
     const int t = blockIdx.x;
     if (t >= gd.n) return;
     const int a = gd.ids[t];
+    const int ia = inst_of(gd, a);
+    gs = at_inst(gs, ia);
+    gd = at_inst(gd, ia);
     const V3<T> ra = ld3(D.pos, a);
     const T h = D.h[a], sup = (T)2.0 * h;
     T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // flt, rho_f, vel_f xyz, pgf xyz
@@ -337,10 +346,12 @@ __global__ __launch_bounds__(kThreads) void k_interaction_dem(Prm<T> p, DemV<T> 
 // kernel/KERNEL_step.py:6-22 (STEP): both only touch the subject's own fields.
 template <typename T, bool INTER, bool STEP>
 __global__ __launch_bounds__(kThreads) void k_fluid_final(Prm<T> p, SphV<T> P, DemV<T> D,
-                                                          GridV<T> gs, GridV<T> gd, T csup) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          GridV<T> gs, GridV<T> gd, T csup,
+                                                          int remap) {
+    const int k = subject_id(remap);
     if (k >= gs.n) return;
     const int i = gs.ids[k];
+    gd = at_inst(gd, inst_of(gs, k));
     V3<T> acc = ld3(P.acc, i);
     if (INTER) {
         const V4<T> me = gs.s[k];
@@ -441,24 +452,29 @@ static int check_csr(const sphdem_dem_t *D, bool wall) {
     return SPHDEM_OK;
 }
 
-template <typename T, bool WALL>
-static void launch_count(const sphdem_dem_t *D, const sphdem_grid_t *g, const void *nb_radius,
-                         double rmax, hipStream_t st) {
-    int *off = WALL ? D->contact_bnd_offset_new : D->contact_dem_offset_new;
-    ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
-    hipLaunchKernelGGL((k_count_contacts<T, WALL>), dim3(blocks_for(D->n + 1, 64)), dim3(64), 0, st,
-                       view<T>(D), view<T>(g), (const T *)nb_radius, (T)rmax, off);
+// The instance of each DEM, when the DEM grid batches several scenes.
+static const int *dem_inst(const sphdem_grid_t *gd) {
+    return gd && gd->n_inst > 1 ? gd->inst : nullptr;
 }
 
 template <typename T, bool WALL>
-static void launch_contact(const sphdem_dem_t *D, const sphdem_grid_t *g, const void *nb_radius,
-                           const void *nb_vel, const void *nb_omega, double rmax, double grav,
-                           double dt, int *flags, hipStream_t st) {
+static void launch_count(const sphdem_dem_t *D, const sphdem_grid_t *g, const int *dinst,
+                         const void *nb_radius, double rmax, hipStream_t st) {
+    int *off = WALL ? D->contact_bnd_offset_new : D->contact_dem_offset_new;
+    ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
+    hipLaunchKernelGGL((k_count_contacts<T, WALL>), dim3(blocks_for(D->n + 1, 64)), dim3(64), 0, st,
+                       view<T>(D), view<T>(g), dinst, (const T *)nb_radius, (T)rmax, off);
+}
+
+template <typename T, bool WALL>
+static void launch_contact(const sphdem_dem_t *D, const sphdem_grid_t *g, const int *dinst,
+                           const void *nb_radius, const void *nb_vel, const void *nb_omega,
+                           double rmax, double grav, double dt, int *flags, hipStream_t st) {
     if (D->n == 0) return;
     ProfScope prof(SPHDEM_PASS_DEM_CONTACT, st);
     hipLaunchKernelGGL((k_contact_force<T, WALL>), dim3(blocks_for(D->n, 64)), dim3(64), 0, st,
-                       view<T>(D), view<T>(g), (const T *)nullptr, (const T *)nb_radius,
-                       (const T *)nb_vel, (const T *)nb_omega, (T)rmax, (T)grav, (T)dt, flags);
+                       view<T>(D), view<T>(g), dinst, (const T *)nb_radius, (const T *)nb_vel,
+                       (const T *)nb_omega, (T)rmax, (T)grav, (T)dt, flags);
 }
 
 template <typename T>
@@ -522,9 +538,10 @@ static void launch_fluid_final(const sphdem_params_t *p, const sphdem_sph_t *P,
                                const sphdem_grid_t *gd, double csup, hipStream_t st) {
     if (gs->n == 0) return;
     ProfScope prof(SPHDEM_PASS_FLUID_FINAL, st);
-    hipLaunchKernelGGL((k_fluid_final<T, INTER, STEP>), dim3(blocks_for(gs->n, kThreads)),
-                       dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs),
-                       view<T>(gd), (T)csup);
+    const int bt = tuned_block(SPHDEM_PASS_FLUID_FINAL);
+    hipLaunchKernelGGL((k_fluid_final<T, INTER, STEP>), dim3(blocks_for(gs->n, bt)), dim3(bt), 0,
+                       st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
+                       (T)csup, tuned_remap(SPHDEM_PASS_FLUID_FINAL));
 }
 
 template <typename T>
@@ -547,7 +564,7 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
     if (all || stage == 0) {
         rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
         if (rc) return rc;
-        launch_count<T, false>(D, gd, D->radius, p->radius_max, st);
+        launch_count<T, false>(D, gd, dem_inst(gd), D->radius, p->radius_max, st);
         {
             ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
             rc = exclusive_scan(D->contact_dem_offset_new, D->contact_dem_offset_new, D->n + 1,
@@ -556,10 +573,10 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         if (rc) return rc;
     }
     if (all || stage == 1) {
-        launch_contact<T, false>(D, gd, D->radius, D->vel, D->omega, p->radius_max, p->g, p->dt,
-                                 w->flags, st);
+        launch_contact<T, false>(D, gd, dem_inst(gd), D->radius, D->vel, D->omega,
+                                 p->radius_max, p->g, p->dt, w->flags, st);
         if (all) launch_copy_csr<T>(D, false, st);
-        launch_count<T, true>(D, gw, W->radius, p->boundary_radius_max, st);
+        launch_count<T, true>(D, gw, dem_inst(gd), W->radius, p->boundary_radius_max, st);
         {
             ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
             rc = exclusive_scan(D->contact_bnd_offset_new, D->contact_bnd_offset_new, D->n + 1,
@@ -568,8 +585,8 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         if (rc) return rc;
     }
     if (all || stage == 2) {
-        launch_contact<T, true>(D, gw, W->radius, W->vel, W->omega, p->boundary_radius_max, 0.0,
-                                p->dt, w->flags, st);
+        launch_contact<T, true>(D, gw, dem_inst(gd), W->radius, W->vel, W->omega,
+                                p->boundary_radius_max, 0.0, p->dt, w->flags, st);
         if (all) launch_copy_csr<T>(D, true, st);
         const double csup = 2.0 * p->h_max;
         rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st);
@@ -617,7 +634,8 @@ int sphdem_count_dem_contacts(const sphdem_params_t *p, sphdem_dem_t *D, const s
     if (rc) return rc;
     SPHDEM_REQUIRE(D->contact_dem_offset_new, "contact_dem_offset_new is NULL");
     SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
-    DISPATCH(p->precision, (launch_count<T, false>(D, gd, D->radius, radius_max, as_stream(stream))));
+    DISPATCH(p->precision, (launch_count<T, false>(D, gd, dem_inst(gd), D->radius, radius_max,
+                                                   as_stream(stream))));
     return check_launch("count_dem_contacts");
 }
 
@@ -629,8 +647,9 @@ int sphdem_force_dem(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_gri
     if (rc) return rc;
     SPHDEM_REQUIRE(flags, "flags buffer is NULL");
     SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
-    DISPATCH(p->precision, (launch_contact<T, false>(D, gd, D->radius, D->vel, D->omega, radius_max,
-                                                     g, p->dt, flags, as_stream(stream))));
+    DISPATCH(p->precision, (launch_contact<T, false>(D, gd, dem_inst(gd), D->radius, D->vel,
+                                                     D->omega, radius_max, g, p->dt, flags,
+                                                     as_stream(stream))));
     return check_launch("force_dem");
 }
 
@@ -640,7 +659,9 @@ int sphdem_count_bnd_contacts(const sphdem_params_t *p, sphdem_dem_t *D, const s
     if (rc) return rc;
     SPHDEM_REQUIRE(W && D->contact_bnd_offset_new, "NULL argument");
     SPHDEM_REQUIRE(gw->n == W->n, "DEM-wall grid was built from a different number of points");
-    DISPATCH(p->precision, (launch_count<T, true>(D, gw, W->radius, radius_max, as_stream(stream))));
+    SPHDEM_REQUIRE(gw->n_inst <= 1, "a batched DEM-wall grid needs sphdem_coupled_step");
+    DISPATCH(p->precision, (launch_count<T, true>(D, gw, nullptr, W->radius, radius_max,
+                                                  as_stream(stream))));
     return check_launch("count_bnd_contacts");
 }
 
@@ -652,8 +673,10 @@ int sphdem_bc_dem(const sphdem_params_t *p, sphdem_dem_t *D, const sphdem_dembnd
     if (rc) return rc;
     SPHDEM_REQUIRE(W && flags, "NULL argument");
     SPHDEM_REQUIRE(gw->n == W->n, "DEM-wall grid was built from a different number of points");
-    DISPATCH(p->precision, (launch_contact<T, true>(D, gw, W->radius, W->vel, W->omega, radius_max,
-                                                    0.0, p->dt, flags, as_stream(stream))));
+    SPHDEM_REQUIRE(gw->n_inst <= 1, "a batched DEM-wall grid needs sphdem_coupled_step");
+    DISPATCH(p->precision, (launch_contact<T, true>(D, gw, nullptr, W->radius, W->vel, W->omega,
+                                                    radius_max, 0.0, p->dt, flags,
+                                                    as_stream(stream))));
     return check_launch("bc_dem");
 }
 

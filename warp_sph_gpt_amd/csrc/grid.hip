@@ -33,6 +33,22 @@ int check_launch(const char *what) {
     return SPHDEM_OK;
 }
 
+// ---------------------------------------------------------------- launch shapes
+namespace {
+sphdem_tuning_t default_tuning() {
+    sphdem_tuning_t t;
+    for (int i = 0; i < SPHDEM_NUM_PASSES; ++i) {
+        t.block[i] = 256;
+        t.xcd_remap[i] = 0;
+    }
+    t.por_x_tile = 4;
+    return t;
+}
+sphdem_tuning_t g_tuning = default_tuning();
+}  // namespace
+
+const sphdem_tuning_t &tuning() { return g_tuning; }
+
 // ---------------------------------------------------------------- timing
 namespace {
 struct Prof {
@@ -151,13 +167,14 @@ __global__ void k_fill_int(int *a, int n, int v) {
 }
 
 template <typename T>
-__global__ void k_grid_key(const T *pos, int n, T inv, int dx, int dy, int dz, int *cells,
-                           int *rank, int *count) {
+__global__ void k_grid_key(const T *pos, int n, T inv, int dx, int dy, int dz, const int *inst,
+                           int *cells, int *rank, int *count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int x = (int)(pos[3 * i] * inv), y = (int)(pos[3 * i + 1] * inv),
               z = (int)(pos[3 * i + 2] * inv);
-    const int key = fold(z, dz) * (dx * dy) + fold(y, dy) * dx + fold(x, dx);
+    int key = fold(z, dz) * (dx * dy) + fold(y, dy) * dx + fold(x, dx);
+    if (inst) key += inst[i] * (dx * dy * dz);
     cells[i] = key;
     rank[i] = atomicAdd(&count[key], 1);
 }
@@ -196,24 +213,26 @@ template <typename T>
 static int grid_build(sphdem_grid_t *g, const void *points, int n, double radius, const void *w,
                       hipStream_t st) {
     ProfScope prof(SPHDEM_PASS_GRID, st);
-    const int ncell = g->dim_x * g->dim_y * g->dim_z;
+    const int ncell = grid_cells(g);
+    const int bt = tuned_block(SPHDEM_PASS_GRID);
+    const int *inst = g->n_inst > 1 ? g->inst : nullptr;
     const T inv = (T)1.0 / (T)radius;
     g->cell_width = radius;
     g->n = n;
-    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, g->cell_hi, ncell, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks_for(ncell, bt)), dim3(bt), 0, st, g->cell_hi, ncell, 0);
     if (n > 0)
-        hipLaunchKernelGGL(k_grid_key<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                           (const T *)points, n, inv, g->dim_x, g->dim_y, g->dim_z, g->point_cells,
-                           g->rank, g->cell_hi);
+        hipLaunchKernelGGL(k_grid_key<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
+                           (const T *)points, n, inv, g->dim_x, g->dim_y, g->dim_z, inst,
+                           g->point_cells, g->rank, g->cell_hi);
     int rc = exclusive_scan(g->cell_hi, g->cell_lo, ncell, g->scan_tmp, st);
     if (rc) return rc;
     if (n > 0)
-        hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            g->point_cells, g->rank, g->cell_lo, n, g->point_ids);
-    hipLaunchKernelGGL(k_grid_fix, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, g->cell_lo,
+    hipLaunchKernelGGL(k_grid_fix, dim3(blocks_for(ncell, bt)), dim3(bt), 0, st, g->cell_lo,
                        g->cell_hi, g->point_ids, ncell);
     if (n > 0)
-        hipLaunchKernelGGL(k_grid_sorted<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_grid_sorted<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            (const T *)points, (const T *)w, g->point_ids, n, (V4<T> *)g->sorted);
     return check_launch("grid_build");
 }
@@ -239,7 +258,7 @@ __global__ void k_neighbour_counts(GridV<T> gs, GridV<T> gn, T r, int *counts) {
     const V4<T> me = gs.s[k];
     const T rr = r * r;
     int c = 0;
-    query_runs(gn, me.x, me.y, me.z, r, [&](int a, int b) {
+    query_runs(at_inst(gn, inst_of(gs, k)), me.x, me.y, me.z, r, [&](int a, int b) {
         for (int s = a; s < b; ++s) {
             const V4<T> q = gn.s[s];
             const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
@@ -253,7 +272,11 @@ static int check_grid(const sphdem_grid_t *g) {
     SPHDEM_REQUIRE(g != nullptr, "grid is NULL");
     SPHDEM_REQUIRE(g->precision == 4 || g->precision == 8, "grid precision must be 4 or 8");
     SPHDEM_REQUIRE(g->dim_x > 0 && g->dim_y > 0 && g->dim_z > 0, "grid dimensions must be positive");
-    SPHDEM_REQUIRE((long)g->dim_x * g->dim_y * g->dim_z < (1L << 30), "grid has too many buckets");
+    SPHDEM_REQUIRE(g->n_inst >= 0, "n_inst must be nonnegative");
+    SPHDEM_REQUIRE((long)g->dim_x * g->dim_y * g->dim_z * (g->n_inst > 1 ? g->n_inst : 1) <
+                       (1L << 30),
+                   "grid has too many buckets");
+    SPHDEM_REQUIRE(g->n_inst <= 1 || g->n == 0 || g->inst, "batched grid needs its inst array");
     SPHDEM_REQUIRE(g->cell_lo && g->cell_hi && g->scan_tmp, "grid bucket buffers are NULL");
     SPHDEM_REQUIRE(g->n == 0 || (g->point_ids && g->sorted), "grid point buffers are NULL");
     return SPHDEM_OK;
@@ -304,6 +327,24 @@ int sphdem_device_count(void) {
 
 int sphdem_scan_scratch(int n) { return scan_scratch(n); }
 
+int sphdem_tuning_get(sphdem_tuning_t *t) {
+    SPHDEM_REQUIRE(t, "tuning is NULL");
+    *t = g_tuning;
+    return SPHDEM_OK;
+}
+
+int sphdem_tuning_set(const sphdem_tuning_t *t) {
+    SPHDEM_REQUIRE(t, "tuning is NULL");
+    for (int i = 0; i < SPHDEM_NUM_PASSES; ++i) {
+        SPHDEM_REQUIRE(t->block[i] == 64 || t->block[i] == 128 || t->block[i] == 256,
+                       "block sizes must be 64, 128 or 256");
+        SPHDEM_REQUIRE(t->xcd_remap[i] == 0 || t->xcd_remap[i] == 1, "xcd_remap must be 0 or 1");
+    }
+    SPHDEM_REQUIRE(t->por_x_tile >= 2 && t->por_x_tile <= 6, "por_x_tile must be in 2..6");
+    g_tuning = *t;
+    return SPHDEM_OK;
+}
+
 int sphdem_exclusive_scan(int *a, int n, int *scratch, void *stream) {
     SPHDEM_REQUIRE(n >= 0, "scan length must be nonnegative");
     SPHDEM_REQUIRE(n == 0 || (a && scratch), "scan buffers are NULL");
@@ -344,7 +385,7 @@ int sphdem_grid_warp_bounds(const sphdem_grid_t *g, int *cell_starts, int *cell_
     int rc = check_grid(g);
     if (rc) return rc;
     SPHDEM_REQUIRE(cell_starts && cell_ends, "bounds buffers are NULL");
-    const int ncell = g->dim_x * g->dim_y * g->dim_z;
+    const int ncell = grid_cells(g);
     hipLaunchKernelGGL(k_warp_bounds, dim3(blocks_for(ncell, 256)), dim3(256), 0, as_stream(stream),
                        g->cell_lo, g->cell_hi, ncell, cell_starts, cell_ends);
     return check_launch("grid_warp_bounds");

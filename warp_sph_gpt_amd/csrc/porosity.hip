@@ -157,10 +157,11 @@ __device__ __forceinline__ void stream_range(T &acc, const V4<T> &me, int a, int
 // 1. near-DEM flag; far subjects get their porosity now.
 template <typename T>
 __global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T> gd, T csup,
-                           int *flag) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                           int *flag, int remap) {
+    const int k = subject_id(remap);
     if (k >= gs.n) return;
     const V4<T> me = gs.s[k];
+    gd = at_inst(gd, inst_of(gs, k));
     bool near = false;
     query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
         for (int s = lo; s < hi; ++s) {
@@ -210,7 +211,7 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
             if (xb < xa) continue;
             const int row = cz * dxy + fold(y, gs.dy) * gs.dx;
             int a1, b1, a2, b2;
-            row_slots(gs.lo, gs.hi, row, gs.dx, xa, xb, a1, b1, a2, b2);
+            row_slots(gs.lo + gs.off, gs.hi + gs.off, row, gs.dx, xa, xb, a1, b1, a2, b2);
             a1 = uniform_i(a1); b1 = uniform_i(b1);
             a2 = uniform_i(a2); b2 = uniform_i(b2);
             stream_range<T, FAST>(flt, me, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
@@ -221,43 +222,48 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
 }
 
 // 2. one wave per 64 consecutive fluid slots.  The flagged lanes are
-// streamed in groups that share a bucket row (y, z) -- a wave of sorted
-// slots can straddle two rows -- and, when a row group is still too wide
-// (folded buckets), in sub-groups of 16 buckets along x; a sub-group that
-// still does not fit (grids narrower than the query) walks its own query
-// per lane.
+// streamed in groups that share an instance and a bucket row (y, z) -- a
+// wave of sorted slots can straddle two rows -- and, when a row group is
+// still too wide (folded buckets), in sub-groups of 2^xt buckets along x; a
+// sub-group that still does not fit (grids narrower than the query) walks
+// its own query per lane.
 template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
                                                     GridV<T> gd, const T *Vs, const int *flag,
-                                                    T csup, T hu, T thr2) {
+                                                    T csup, T hu, T thr2, int xt, int remap) {
     __shared__ V4<T> stage[256];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = subject_id(remap);
     const int lane = threadIdx.x & 63;
     V4<T> *wbuf = stage + (threadIdx.x & ~63);
     const bool active = k < gs.n && flag[k] != 0;
     const unsigned long long live = __ballot(active);
     if (live == 0ull) return;
     const int lead = (int)(k & ~63) + __ffsll((long long)live) - 1;
-    const V4<T> me = gs.s[active ? k : lead];
+    const int ks = active ? k : lead;
+    const V4<T> me = gs.s[ks];
+    const int ik = inst_of(gs, ks);
+    gd = at_inst(gd, ik);
     const T h = hu, inv_h = (T)1.0 / hu, inv = gs.inv;
-    const int ky = (int)(me.y * inv), kz = (int)(me.z * inv), kx = ((int)(me.x * inv)) >> 4;
+    const int ky = (int)(me.y * inv), kz = (int)(me.z * inv), kx = ((int)(me.x * inv)) >> xt;
     T flt = (T)0;
     unsigned long long todo = live;
     while (todo) {
         const int ld = __ffsll((long long)todo) - 1;
-        const int ly = __shfl(ky, ld, 64), lz = __shfl(kz, ld, 64);
-        const unsigned long long grp = __ballot(active && ky == ly && kz == lz) & todo;
+        const int ly = __shfl(ky, ld, 64), lz = __shfl(kz, ld, 64), li = __shfl(ik, ld, 64);
+        const unsigned long long grp =
+            __ballot(active && ky == ly && kz == lz && ik == li) & todo;
         const bool in = (grp >> lane) & 1ull;
-        if (!stream_group<T, FAST>(flt, me, in, gs, Vs, wbuf, lane, csup, thr2, h, inv_h)) {
+        const GridV<T> gg = at_inst(gs, uniform_i(li));
+        if (!stream_group<T, FAST>(flt, me, in, gg, Vs, wbuf, lane, csup, thr2, h, inv_h)) {
             unsigned long long rest = grp;
             while (rest) {
                 const int l2 = __ffsll((long long)rest) - 1;
                 const int lx = __shfl(kx, l2, 64);
                 const unsigned long long sub = __ballot(kx == lx) & rest;
                 const bool in2 = (sub >> lane) & 1ull;
-                if (!stream_group<T, FAST>(flt, me, in2, gs, Vs, wbuf, lane, csup, thr2, h,
+                if (!stream_group<T, FAST>(flt, me, in2, gg, Vs, wbuf, lane, csup, thr2, h,
                                            inv_h) && in2) {
-                    query_runs(gs, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+                    query_runs(at_inst(gs, ik), me.x, me.y, me.z, csup, [&](int lo, int hi) {
                         for (int s = lo; s < hi; ++s) {
                             const V4<T> q = gs.s[s];
                             add_term<T, FAST>(flt, me, q.x, q.y, q.z, Vs[s], thr2, h, inv_h);
@@ -303,17 +309,19 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
     if (n == 0) return SPHDEM_OK;
     ProfScope prof(SPHDEM_PASS_POROSITY, st);
     const T cs = (T)csup;
-    hipLaunchKernelGGL(k_por_mark<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st, prm<T>(p),
-                       view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), cs, w->fl_flag);
+    const int bt = tuned_block(SPHDEM_PASS_POROSITY), rm = tuned_remap(SPHDEM_PASS_POROSITY);
+    const int xt = tuning().por_x_tile;
+    hipLaunchKernelGGL(k_por_mark<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
+                       view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), cs, w->fl_flag, rm);
     const T thr2 = sqrt_threshold<T>(cs);
     if (p->fast_math)
-        hipLaunchKernelGGL((k_por_stream<T, true>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL((k_por_stream<T, true>), dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2);
+                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm);
     else
-        hipLaunchKernelGGL((k_por_stream<T, false>), dim3(blocks_for(n, 256)), dim3(256), 0, st,
+        hipLaunchKernelGGL((k_por_stream<T, false>), dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2);
+                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm);
     return check_launch("porosity");
 }
 template int launch_porosity_tiled<float>(const sphdem_params_t *, const sphdem_sph_t *,

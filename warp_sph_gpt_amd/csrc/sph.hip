@@ -29,11 +29,14 @@ constexpr int kThreads = 256;
 template <typename T, bool SHEP, bool DENS, bool WORK>
 __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P, BndV<T> B,
                                                             GridV<T> gs, GridV<T> gb,
-                                                            WorkV<T> w) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                                                            WorkV<T> w, int remap) {
+    const int k = subject_id(remap);
     if (k >= gs.n) return;
     const V4<T> me = gs.s[k];
     const int i = gs.ids[k];
+    const int ik = inst_of(gs, k);
+    gs = at_inst(gs, ik);
+    gb = at_inst(gb, ik);
     const T sup = p.support, ss = sup * sup, h = p.h;
     T flt = (T)0, rho = (T)0;
     query_runs(gs, me.x, me.y, me.z, sup, [&](int a, int b) {
@@ -79,11 +82,15 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
 // and the sorted copies the force pass reads.
 template <typename T, bool WORK>
 __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, BndV<T> B,
-                                                           GridV<T> gs, GridV<T> gb, WorkV<T> w) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                                                           GridV<T> gs, GridV<T> gb, WorkV<T> w,
+                                                           int remap) {
+    const int k = subject_id(remap);
     if (k >= gb.n) return;
     const V4<T> me = gb.s[k];
     const int bi = gb.ids[k];
+    const int ik = inst_of(gb, k);
+    gs = at_inst(gs, ik);
+    gb = at_inst(gb, ik);
     const T sup = p.support, ss = sup * sup, h = p.h;
     T rho = (T)0;
     auto acc = [&](const GridV<T> &g) {
@@ -121,11 +128,15 @@ __global__ void k_pres(Prm<T> p, const T *rho, T *pres, int n) {
 //  PGF also leaves (pgf, rho) and m/rho in sorted order for the coupling.
 template <typename T, bool ACC, bool PGF, bool PUBLISH>
 __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, BndV<T> B,
-                                                          GridV<T> gs, GridV<T> gb, WorkV<T> w) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          GridV<T> gs, GridV<T> gb, WorkV<T> w,
+                                                          int remap) {
+    const int k = subject_id(remap);
     if (k >= gs.n) return;
     const V4<T> me = gs.s[k];
     const int i = gs.ids[k];
+    const int ik = inst_of(gs, k);
+    gs = at_inst(gs, ik);
+    gb = at_inst(gb, ik);
     const T rhoi = w.fl_rho[k], presi = w.fl_pres[k];
     const V3<T> vi = xyz(w.fl_vel[k]);
     const T sup = p.support, ss = sup * sup, h = p.h, r2min = (T)1.0e-12;
@@ -245,9 +256,10 @@ static void launch_fluid_density(const sphdem_params_t *p, const sphdem_sph_t *P
     ProfScope prof(SPHDEM_PASS_FLUID_DENSITY, st);
     WorkV<T> wv{};
     if (w) wv = view<T>(w);
-    hipLaunchKernelGGL((k_fluid_density<T, SHEP, DENS, WORK>), dim3(blocks_for(gs->n, kThreads)),
-                       dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
-                       view<T>(gb), wv);
+    const int bt = tuned_block(SPHDEM_PASS_FLUID_DENSITY);
+    hipLaunchKernelGGL((k_fluid_density<T, SHEP, DENS, WORK>), dim3(blocks_for(gs->n, bt)),
+                       dim3(bt), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
+                       view<T>(gb), wv, tuned_remap(SPHDEM_PASS_FLUID_DENSITY));
 }
 
 template <typename T, bool WORK>
@@ -258,8 +270,10 @@ static void launch_wall_density(const sphdem_params_t *p, const sphdem_sph_t *P,
     ProfScope prof(SPHDEM_PASS_WALL_DENSITY, st);
     WorkV<T> wv{};
     if (w) wv = view<T>(w);
-    hipLaunchKernelGGL((k_wall_density<T, WORK>), dim3(blocks_for(gb->n, kThreads)), dim3(kThreads),
-                       0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb), wv);
+    const int bt = tuned_block(SPHDEM_PASS_WALL_DENSITY);
+    hipLaunchKernelGGL((k_wall_density<T, WORK>), dim3(blocks_for(gb->n, bt)), dim3(bt), 0, st,
+                       prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb), wv,
+                       tuned_remap(SPHDEM_PASS_WALL_DENSITY));
 }
 
 template <typename T, bool ACC, bool PGF, bool PUBLISH>
@@ -268,9 +282,10 @@ static void launch_fluid_force(const sphdem_params_t *p, const sphdem_sph_t *P,
                                const sphdem_grid_t *gb, const sphdem_work_t *w, hipStream_t st) {
     if (gs->n == 0) return;
     ProfScope prof(SPHDEM_PASS_FLUID_FORCE, st);
-    hipLaunchKernelGGL((k_fluid_force<T, ACC, PGF, PUBLISH>), dim3(blocks_for(gs->n, kThreads)),
-                       dim3(kThreads), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
-                       view<T>(gb), view<T>(w));
+    const int bt = tuned_block(SPHDEM_PASS_FLUID_FORCE);
+    hipLaunchKernelGGL((k_fluid_force<T, ACC, PGF, PUBLISH>), dim3(blocks_for(gs->n, bt)),
+                       dim3(bt), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
+                       view<T>(gb), view<T>(w), tuned_remap(SPHDEM_PASS_FLUID_FORCE));
 }
 
 template <typename T>

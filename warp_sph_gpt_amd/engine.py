@@ -8,8 +8,11 @@ SPHDEM_OneStep -- with the same kernels as simulation.py, but:
     finished rows are copied into the read side instead of swapping, and a
     device flag records overflow (checked by `check()`);
   * every buffer address is fixed after construction, so one step can be
-    captured in a hipGraph and replayed (`run(..., graph=True)`): two
-    graphs, with and without the Shepard refresh.
+    captured in a hipGraph and replayed (`capture()`): two graphs, with and
+    without the Shepard refresh.
+With `instances` the structs hold several independent scenes back to back
+(ensemble.py): every grid is built batched, and each kernel queries the
+buckets of its subject's own scene only.
 """
 from __future__ import annotations
 
@@ -24,7 +27,10 @@ from .simulation import solver_params
 
 class CoupledEngine:
     def __init__(self, solv, P_sph, P_bnd, P_dem=None, P_dem_bnd=None, csr_per_dem: int = 64,
-                 grid_slice: int | None = None, fast_math: bool | None = None):
+                 grid_slice: int | None = None, fast_math: bool | None = None,
+                 instances: dict | None = None):
+        """instances: {"n": scenes, "fluid"/"wall"/"dem"/"dem_wall": int32
+        nondecreasing scene number of each particle} for a batched state."""
         self.solv = solv
         self.P, self.B, self.D, self.W = P_sph, P_bnd, P_dem, P_dem_bnd
         dev, dtype = P_sph.device, P_sph.dtype
@@ -33,6 +39,10 @@ class CoupledEngine:
         self.coupled = bool(solv.dem_enable and P_dem is not None)
         self.gs = HashGrid(gsl, gsl, gsl, device=dev)
         self.gb = HashGrid(gsl, gsl, gsl, device=dev)
+        self.instances = instances
+        if instances is not None:
+            self.gs.instances(instances["fluid"], instances["n"])
+            self.gb.instances(instances["wall"], instances["n"])
         self.gs.reserve(P_sph.n, dtype)
         self.gb.build(P_bnd.pos, solv.support, weights=P_bnd.m)
         self.ws = Workspace(dev, dtype)
@@ -43,6 +53,9 @@ class CoupledEngine:
             self.scales = validate_dem_particles(solv, P_dem, P_dem_bnd)
             self.gd = HashGrid(gsl, gsl, gsl, device=dev)
             self.gw = HashGrid(gsl, gsl, gsl, device=dev)
+            if instances is not None:
+                self.gd.instances(instances["dem"], instances["n"])
+                self.gw.instances(instances["dem_wall"], instances["n"])
             self.gd.reserve(nd, dtype)
             self.gw.build(P_dem_bnd.pos, max(2.0 * self.scales[1], solv.support))
             self.r_dem = max(2.0 * self.scales[0], 2.0 * self.scales[2])

@@ -8,6 +8,11 @@ dim_x*dim_y*dim_z buckets, stable sort by key, query order z-outer/y/x).
 The build runs on the GPU (libsphdem: keys, atomic counting sort, device
 scan, per-bucket ordering); all buffers are torch tensors owned here.
 
+A grid can also batch independent scenes ("instances"): with
+`instances(inst, n_inst)` every point carries its instance number (a
+nondecreasing int32 array), each instance gets its own dim_x*dim_y*dim_z
+buckets, and a query by a point of instance k sees instance k only.
+
 Beyond Warp's interface the grid keeps a sorted copy of the points with a
 fourth column (`weights`, e.g. particle mass), which the neighbour kernels
 read contiguously.  Kernels read neighbour positions from that copy, i.e.
@@ -36,12 +41,8 @@ class HashGrid:
         self.device = torch.device(device if device is not None else "cuda")
         if self.device.type != "cuda":
             raise ValueError("HashGrid lives on a GPU device")
-        ncell = self.dim_x * self.dim_y * self.dim_z
-        self.ncell = ncell
-        self.cell_lo = torch.zeros(ncell, dtype=torch.int32, device=self.device)
-        self.cell_hi = torch.zeros(ncell, dtype=torch.int32, device=self.device)
-        self._scan_tmp = torch.zeros(_lib.load().sphdem_scan_scratch(ncell), dtype=torch.int32,
-                                     device=self.device)
+        self.n_inst, self.inst = 1, None
+        self._alloc_buckets()
         self._cap = 0
         self._cells = self._ids = self._rank = self._sorted = None
         self.n = 0
@@ -49,6 +50,31 @@ class HashGrid:
         self.dtype = torch.float32
         self.points = None
         self._w_tag = None
+
+    def _alloc_buckets(self) -> None:
+        ncell = self.dim_x * self.dim_y * self.dim_z * self.n_inst
+        self.ncell = ncell
+        self.cell_lo = torch.zeros(ncell, dtype=torch.int32, device=self.device)
+        self.cell_hi = torch.zeros(ncell, dtype=torch.int32, device=self.device)
+        self._scan_tmp = torch.zeros(_lib.load().sphdem_scan_scratch(ncell), dtype=torch.int32,
+                                     device=self.device)
+
+    def instances(self, inst: torch.Tensor, n_inst: int):
+        """Batch n_inst scenes: inst[i] (int32, nondecreasing, on the grid's
+        device) is the instance of point i in every later build."""
+        n_inst = int(n_inst)
+        if n_inst < 1:
+            raise ValueError("n_inst must be positive")
+        if inst.dtype != torch.int32 or inst.ndim != 1 or inst.device != self.device:
+            raise ValueError("inst must be a 1-D int32 tensor on the grid's device")
+        if inst.numel() and (int(inst.min()) < 0 or int(inst.max()) >= n_inst
+                             or bool((inst[1:] < inst[:-1]).any())):
+            raise ValueError("inst must be nondecreasing values in [0, n_inst)")
+        self.inst = inst.contiguous()
+        if n_inst != self.n_inst:
+            self.n_inst = n_inst
+            self._alloc_buckets()
+        return self
 
     @property
     def id(self):
@@ -78,6 +104,8 @@ class HashGrid:
         g.cell_hi = self.cell_hi.data_ptr()
         g.sorted = self._sorted.data_ptr() if self._sorted is not None else None
         g.scan_tmp = self._scan_tmp.data_ptr()
+        g.n_inst = self.n_inst
+        g.inst = self.inst.data_ptr() if self.inst is not None and self.inst.numel() else None
         return g
 
     def build(self, points: torch.Tensor, radius: float, weights: torch.Tensor | None = None):
@@ -91,6 +119,8 @@ class HashGrid:
         if not radius > 0.0:
             raise ValueError("grid radius must be positive")
         n = int(points.shape[0])
+        if self.inst is not None and self.inst.numel() != n:
+            raise ValueError(f"inst has {self.inst.numel()} entries for {n} points")
         self.dtype = points.dtype
         _lib.precision_of(self.dtype)
         self.reserve(n, self.dtype)

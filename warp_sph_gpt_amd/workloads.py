@@ -12,7 +12,14 @@
       not place them all the box top is raised by 0.05 m and drawing
       continues (the top ends at z = 2.35 m for the default seed).
       DEM-wall spacing 0.02 (14,688), dt=2.5e-5, 500 steps.
-All: tank 2 x 0.4 x 1 m, fluid block 0.5 x 0.4 x 0.5 m at the origin,
+  c3  ensemble of 64 independent dam breaks at dx=0.01, dt=5e-5, 500 steps:
+      instance k draws (numpy default_rng(seed + k)) a fluid block length
+      Lx ~ U[0.3, 0.7] and height Lz ~ U[0.3, 0.6] (rounded to dx; full
+      0.4 m depth), N_dem ~ U{0..100} spheres with R ~ U[0.015, 0.03] placed
+      by random sequential addition above its free surface in
+      [0.04, Lx-0.04] x [0.04, 0.36] x [Lz+0.06, Lz+0.36], its own 3-layer
+      tank wall (180,456) and DEM-wall spheres (spacing 0.02, 14,688).
+All: tank 2 x 0.4 x 1 m, fluid block 0.5 x 0.4 x 0.5 m at the origin (c0-c2),
 3-layer dummy wall, h = 1.3 dx, rho0 = 1000, DEM h = 0.08, K=2e4, eta=4,
 mu=0.3, rho_s=2500, DEM-wall radius 0.01.
 """
@@ -80,4 +87,55 @@ def c2(device="cuda:0", seed=20262, n_dem=2000) -> Workload:
     return _build("c2", gen, 500, dem_pos=pos, note=note, device=device)
 
 
+@dataclass
+class Ensemble:
+    name: str
+    solv: Solv
+    scenes: list     # [(SPHptl, BNDptl, DEMptl | None, DEMBNDptl | None)] on CPU
+    steps: int
+    params: list     # per instance: dict(Lx, Lz, n_dem, seed)
+    note: str = ""
+
+    @property
+    def n_fluid(self) -> int:
+        return sum(sc[0].n for sc in self.scenes)
+
+
+def ensemble_scene(seed: int, dx: float = 0.01, dt: float = 5.0e-5, max_dem: int = 100,
+                   n_dem: int | None = None):
+    """One instance of the ensemble: (gen, (P, B, D, W), params); `n_dem`
+    overrides the drawn sphere count."""
+    rng = np.random.default_rng(seed)
+    lx = round(float(rng.uniform(0.3, 0.7)) / dx) * dx
+    lz = round(float(rng.uniform(0.3, 0.6)) / dx) * dx
+    drawn = int(rng.integers(0, max_dem + 1))
+    n_dem = drawn if n_dem is None else int(n_dem)
+    gen = CoupledGenerationConfig(device="cpu", dx=dx, dt=dt, fluid_width=lx, fluid_height=lz)
+    P, B = DamPtlGeneration(gen).build()
+    D = W = None
+    if n_dem:
+        lo, hi = (0.04, 0.04, lz + 0.06), (lx - 0.04, 0.36, lz + 0.36)
+        pos, rad, _ = rsa_spheres(n_dem, lambda r, m: r.uniform(0.015, 0.03, m), lo, hi,
+                                  seed + 7919, attempts=50_000, grow_z=0.02, max_grow=10,
+                                  r_max=0.03)
+        check_dem_inside(pos, rad, gen)
+        D, W = DEMPtlGeneration(gen).build(dem_pos=pos, dem_radius=rad)
+    return gen, (P, B, D, W), {"Lx": lx, "Lz": lz, "n_dem": n_dem, "seed": seed}
+
+
+def c3(device="cuda:0", seed=20263, n_inst=64, steps=500, dx=0.01) -> Ensemble:
+    scenes, params, gen = [], [], None
+    for k in range(n_inst):
+        gen, sc, prm = ensemble_scene(seed + k, dx=dx)
+        scenes.append(sc)
+        params.append(prm)
+    solv = runtime_solv(gen, device)
+    solv.n_steps = steps
+    nf = sum(sc[0].n for sc in scenes)
+    nd = sum(p["n_dem"] for p in params)
+    note = f"{n_inst} instances, {nf} fluid, {nd} DEM, seeds {seed}..{seed + n_inst - 1}"
+    return Ensemble("c3", solv, scenes, steps, params, note)
+
+
 WORKLOADS = {"c0": c0, "c1": c1, "c2": c2}
+ENSEMBLES = {"c3": c3}
