@@ -288,6 +288,86 @@ int sphdem_reduce_f64(int precision, const void *sph_pos, const void *sph_vel,
                       const void *dem_m, const void *dem_inertia, int nd, double *out,
                       void *stream);
 
+/* ---- learned surrogate (no reference counterpart) ---------------------- */
+/* A two-round message-passing network over the SPH neighbourhoods of the
+ * fluid and wall grids (width 64, ReLU), BF16 operands on MFMA with FP32
+ * accumulation; it stands in for ten WCSPH substeps.  All node arrays are in
+ * the fluid grid's sorted order, row-major [N x 64] (bf16 arrays as uint16
+ * bit patterns).  Weights are FP32 [K x 64]; sphdem_mpnn_pack writes their
+ * bf16 MFMA B-operand fragments for W (Bp) and W^T (BTp), each
+ * (K/16)*4*64*4 uint16.  FP32 grids and params only. */
+int sphdem_mpnn_pack(const float *W, int K, void *Bp, void *BTp, void *stream);
+int sphdem_mpnn_to_bf16(const float *x, int n, void *y, void *stream);
+/* Y = resid + act(X W + b), X = [X0 (K0 wide) | X1 (K1 wide)] bf16; Yf
+ * (FP32) and/or Yb (bf16) receive Y.  b, resid, X1, Yf, Yb may be NULL. */
+int sphdem_mpnn_dense_fwd(const void *X0, int K0, const void *X1, int K1, int rows,
+                          const void *Bp, const float *b, int relu, const float *resid,
+                          float *Yf, void *Yb, void *stream);
+/* dX (+)= dZ W^T with dZ = dY * [Ymask > 0] (Ymask NULL: no activation). */
+int sphdem_mpnn_dense_bwd_x(const float *dY, const float *Ymask, int rows, const void *BTp,
+                            float *dX0, int K0, float *dX1, int K1, int accumulate,
+                            void *stream);
+/* gW += X^T dZ, gb += column sums of dZ (gb may be NULL). */
+int sphdem_mpnn_dense_bwd_w(const float *dY, const float *Ymask, const void *X0, int K0,
+                            const void *X1, int K1, int rows, float *gW, float *gb,
+                            void *stream);
+/* s_i = (1/64) sum_j relu(relu(P_i + Q_j + e_ij C + c1) W2 + c2) over the
+ * fluid and wall neighbours j of fluid node i (1e-12 < r^2 < support^2);
+ * Q_j = Qw for walls.  fvel: [N x 4] sorted fluid velocity (from
+ * sphdem_mpnn_features); wvel: [nb x 4] sorted wall velocity or NULL. */
+int sphdem_mpnn_edge_fwd(const sphdem_params_t *p, const sphdem_grid_t *gs,
+                         const sphdem_grid_t *gb, const void *fvel, const void *wvel,
+                         const float *P, const void *Q, const void *Qw, const void *Cp,
+                         const float *c1, const void *W2p, const float *c2, float *s,
+                         void *s_bf, void *stream);
+/* Backward of the edge pass for ds = dL/ds: writes dP, adds into dQ (zeroed
+ * by the caller) and into the parameter gradients gC, gc1, gW2, gc2, gQw. */
+int sphdem_mpnn_edge_bwd(const sphdem_params_t *p, const sphdem_grid_t *gs,
+                         const sphdem_grid_t *gb, const void *fvel, const void *wvel,
+                         const float *P, const void *Q, const void *Qw, const void *Cp,
+                         const float *c1, const void *W2p, const void *W2Tp, const float *c2,
+                         const float *ds, float *dP, float *dQ, float *gC, float *gc1,
+                         float *gW2, float *gc2, float *gQw, void *stream);
+/* Node features [N x 16] bf16: v/U, a/A, 100 (rho/rho0 - 1) (U = 1 m/s,
+ * A = 9.81 m/s^2) and the sorted velocity [N x 4]. */
+int sphdem_mpnn_features(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_grid_t *gs,
+                         void *nfeat, void *fvel, void *stream);
+/* Training target [N x 6] (sorted): ((v1 - v0 - T a0) / (A T),
+ * (x1 - x0 - T v0 - T^2 a0 / 2) / (A T^2 / 2)); arrays at original index. */
+int sphdem_mpnn_targets(const sphdem_grid_t *gs, const float *x0, const float *v0,
+                        const float *a0, const float *x1, const float *v1, double T,
+                        float *target, void *stream);
+/* loss += mean of (out[:, 0:6] - target)^2 (FP64, device), dout = its gradient. */
+int sphdem_mpnn_loss(const float *out, const float *target, int n, double *loss, float *dout,
+                     void *stream);
+/* One surrogate call: v += T a + A T out[0:3], x += T v + T^2 a / 2 + A T^2 out[3:6] / 2. */
+int sphdem_mpnn_apply(const sphdem_params_t *p, sphdem_sph_t *P, const sphdem_grid_t *gs,
+                      const float *out, double T, void *stream);
+/* Start of a surrogate call: the fused SPH passes of step `step` (Shepard
+ * cadence, density, EOS, acceleration, -grad p), the DEM coupling sums of
+ * Kernel_interaction_dem and the drag reaction added to P.acc (with the
+ * fluid porosity of the last plain step).  D may be NULL (no DEM). */
+int sphdem_surrogate_prep(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                          sphdem_dem_t *D, const sphdem_grid_t *gs, const sphdem_grid_t *gb,
+                          const sphdem_grid_t *gd, int step, const sphdem_work_t *w,
+                          void *stream);
+/* n_sub DEM substeps of a surrogate call, all in one workgroup: contacts
+ * (DEM-DEM candidates from gd as built at the call's start, queried with the
+ * radius widened by `skin` and read at their current positions; DEM-wall
+ * from the static gw), the drag and pressure force held from the call's
+ * start, and the integration of Kernel_step_dem.  The contact histories go
+ * back to the old and new CSR as stage -1 of sphdem_coupled_step leaves them.
+ * `scratch` holds sphdem_dem_substeps_scratch(n, precision) bytes.  flags
+ * bit 0: a CSR or history row overflowed; bit 1: a DEM moved farther than
+ * `skin` from where gd was built (the candidate sets may then be short). */
+long sphdem_dem_substeps_scratch(int n, int precision);
+int sphdem_dem_substeps(const sphdem_params_t *p, sphdem_dem_t *D, sphdem_dembnd_t *W,
+                        const sphdem_grid_t *gd, const sphdem_grid_t *gw, int n_sub, double skin,
+                        void *scratch, const sphdem_work_t *w, void *stream);
+/* Adam step `step` (>= 1) on FP32 parameters. */
+int sphdem_adam(float *param, const float *grad, float *m, float *v, int n, double lr,
+                double beta1, double beta2, double eps, int step, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

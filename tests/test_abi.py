@@ -16,7 +16,8 @@ HEADER = (ROOT / "include" / "sphdem.h").read_text()
 
 
 def declared_functions():
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(sphdem_\w+)\s*\(", HEADER, re.M)))
+    return sorted(set(re.findall(r"^(?:int|long|const char \*)\s*(sphdem_\w+)\s*\(", HEADER,
+                                 re.M)))
 
 
 def struct_fields(name):

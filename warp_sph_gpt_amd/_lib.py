@@ -88,6 +88,37 @@ _SIGNATURES = {
     "sphdem_device_count": ([], c_int),
     "sphdem_scan_scratch": ([c_int], c_int),
     "sphdem_tuning_get": ([POINTER(TuningStruct)], c_int),
+    "sphdem_mpnn_pack": ([c_void_p, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_to_bf16": ([c_void_p, c_int, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_dense_fwd": ([c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int,
+                               c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_dense_bwd_x": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                 c_int, c_int, c_void_p], c_int),
+    "sphdem_mpnn_dense_bwd_w": ([c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int,
+                                 c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_edge_fwd": ([POINTER(Params), POINTER(GridStruct), POINTER(GridStruct), c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_edge_bwd": ([POINTER(Params), POINTER(GridStruct), POINTER(GridStruct), c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_features": ([POINTER(Params), POINTER(SphView), POINTER(GridStruct), c_void_p,
+                              c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_targets": ([POINTER(GridStruct), c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_double, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_loss": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_apply": ([POINTER(Params), POINTER(SphView), POINTER(GridStruct), c_void_p,
+                           c_double, c_void_p], c_int),
+    "sphdem_surrogate_prep": ([POINTER(Params), POINTER(SphView), POINTER(BndView),
+                               POINTER(DemView), POINTER(GridStruct), POINTER(GridStruct),
+                               POINTER(GridStruct), c_int, POINTER(WorkStruct), c_void_p], c_int),
+    "sphdem_dem_substeps_scratch": ([c_int, c_int], ctypes.c_long),
+    "sphdem_dem_substeps": ([POINTER(Params), POINTER(DemView), POINTER(DemBndView),
+                             POINTER(GridStruct), POINTER(GridStruct), c_int, c_double, c_void_p,
+                             POINTER(WorkStruct), c_void_p], c_int),
+    "sphdem_adam": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
+                     c_double, c_int, c_void_p], c_int),
     "sphdem_tuning_set": ([POINTER(TuningStruct)], c_int),
     "sphdem_exclusive_scan": ([c_void_p, c_int, c_void_p, c_void_p], c_int),
     "sphdem_grid_build": ([POINTER(GridStruct), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
@@ -327,3 +358,15 @@ def set_tuning(cfg: dict) -> None:
         t.xcd_remap[i] = int(cfg.get("xcd_remap", {}).get(n, cur["xcd_remap"][n]))
     t.por_x_tile = int(cfg.get("por_x_tile", cur["por_x_tile"]))
     call("sphdem_tuning_set", t)
+
+
+def fptr(t):
+    """Raw device pointer of a contiguous CUDA tensor of any dtype (None
+    passes through): for the surrogate's bf16 (int16) and FP32 buffers."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"buffer must live on a GPU device (got {t.device})")
+    if not t.is_contiguous():
+        raise ValueError("buffer must be contiguous")
+    return c_void_p(t.data_ptr())

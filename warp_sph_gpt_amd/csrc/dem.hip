@@ -598,6 +598,245 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
     return check_launch("coupled_step");
 }
 
+// ---------------------------------------------------------------- surrogate support
+// DEM loads of a substep plus the coupling force frozen at a surrogate
+// call's start (drag and pressure force are zero for a dry sphere).
+template <typename T>
+__global__ void k_add_coupling(DemV<T> D) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= D.n) return;
+    st3(D.force, a, (ld3(D.force, a) + ld3(D.drag, a)) + ld3(D.pressure_force, a));
+}
+
+// Start of a surrogate call: the SPH passes (density, EOS, acceleration with
+// -grad p), the DEM coupling sums (drag, pressure force) and the drag
+// reaction added to the fluid acceleration (with the porosity last computed
+// by a plain step).  Leaves P.acc = the acceleration the network reads.
+template <typename T>
+static int surrogate_prep(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
+                          const sphdem_dem_t *D, const sphdem_grid_t *gs, const sphdem_grid_t *gb,
+                          const sphdem_grid_t *gd, int step, const sphdem_work_t *w,
+                          hipStream_t st) {
+    const bool coupled = D && D->n > 0;
+    int rc = sph_stage<T>(p, P, B, gs, gb, step, coupled, w, st);
+    if (rc || !coupled) return rc;
+    launch_interaction_dem<T>(p, D, gs, gd, w, st);
+    launch_fluid_final<T, true, false>(p, P, D, gs, gd, 2.0 * p->h_max, st);
+    return check_launch("surrogate_prep");
+}
+
+// n_sub DEM substeps of a surrogate call in ONE workgroup (all DEM, the
+// substeps separated by workgroup barriers): per substep the DEM-DEM
+// contacts (candidates from the DEM grid built at the call's start, queried
+// with the radius widened by `skin` and read at their current positions),
+// the DEM-wall contacts (static walls), the coupling force held from the
+// call's start, and the integration of Kernel_step_dem.  Contact histories
+// live in fixed-width rows (kEll per DEM and kind) in `scratch` and go back
+// to the CSR (old and new, as stage -1 leaves them) at the end.  Loads are
+// the same as the plain DEM stage's (dem_contact, gravity, last-match
+// history); flags bit 0: CSR or row overflow, bit 1: a DEM moved farther
+// than `skin` from where the grid was built.
+constexpr int kEll = 32;
+constexpr int kSubThreads = 1024;
+
+template <typename T>
+__device__ __forceinline__ void block_scan_rows(const int *cnt, int n, int *off, int *total) {
+    __shared__ int part[kSubThreads];
+    const int t = threadIdx.x;
+    const int chunk = (n + kSubThreads - 1) / kSubThreads;
+    const int a0 = min(n, t * chunk), a1 = min(n, a0 + chunk);
+    int sum = 0;
+    for (int a = a0; a < a1; ++a) sum += cnt[a];
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < kSubThreads; d <<= 1) {
+        const int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int a = a0; a < a1; ++a) {
+        off[a] = run;
+        run += cnt[a];
+    }
+    if (t == kSubThreads - 1) *total = part[t];
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> D, WallV<T> W,
+                                                              GridV<T> gd, GridV<T> gw,
+                                                              const int *dinst, int n_sub, T skin,
+                                                              T *pos0, int *cnt, int *ids,
+                                                              T *tang, int *flags) {
+    const int n = D.n;
+    // cnt/ids/tang: [side 0/1][kind 0 = dem, 1 = wall][n][kEll]
+    auto row = [&](int side, int kind, int a) { return ((side * 2 + kind) * n + a); };
+    // 0. CSR (old) -> rows of side 0; positions the grid was built from
+    for (int a = threadIdx.x; a < n; a += kSubThreads) {
+        st3(pos0, a, ld3(D.pos, a));
+        for (int kind = 0; kind < 2; ++kind) {
+            const int *off = kind ? D.bnd_off_old : D.dem_off_old;
+            const int *id = kind ? D.bnd_id_old : D.dem_id_old;
+            const T *tg = kind ? D.bnd_tang_old : D.dem_tang_old;
+            const int o0 = off[a], o1 = off[a + 1];
+            int c = 0;
+            for (int k = o0; k < o1; ++k) {
+                if (c == kEll) {
+                    atomicOr(flags, 1);
+                    break;
+                }
+                const int r = row(0, kind, a) * kEll + c;
+                ids[r] = id[k];
+                st3(tang, r, ld3(tg, k));
+                ++c;
+            }
+            cnt[row(0, kind, a)] = c;
+        }
+    }
+    __syncthreads();
+    for (int sub = 0; sub < n_sub; ++sub) {
+        const int cur = sub & 1, nxt = cur ^ 1;
+        // A. loads (reads neighbours' current state; writes force/torque and rows of nxt)
+        for (int a = threadIdx.x; a < n; a += kSubThreads) {
+            const int ia = dinst ? dinst[a] : 0;
+            const V3<T> ra = ld3(D.pos, a);
+            const T Ra = D.radius[a];
+            const V3<T> va = ld3(D.vel, a), wa = ld3(D.omega, a);
+            const T K = D.K[a], eta = D.eta[a], mu = D.mu[a];
+            // DEM-DEM loads start from gravity; the wall loads are summed
+            // apart and added, as the two plain kernels do
+            V3<T> fk[2] = {mk3<T>(0, 0, -D.m[a] * p.g), mk3<T>(0, 0, 0)};
+            V3<T> tk[2] = {mk3<T>(0, 0, 0), mk3<T>(0, 0, 0)};
+            for (int kind = 0; kind < 2; ++kind) {
+                V3<T> &force = fk[kind], &torque = tk[kind];
+                const GridV<T> g = at_inst(kind ? gw : gd, ia);
+                const T *nr = kind ? W.radius : D.radius;
+                const T *nv = kind ? W.vel : D.vel;
+                const T *nw = kind ? W.omega : D.omega;
+                const int rc = row(cur, kind, a), rn = row(nxt, kind, a);
+                const int c0 = cnt[rc];
+                int out = 0;
+                const T reach = Ra + (kind ? p.boundary_radius_max : p.radius_max + skin);
+                query_runs(g, ra.x, ra.y, ra.z, reach, [&](int lo, int hi) {
+                    for (int s = lo; s < hi; ++s) {
+                        const int b = g.ids[s];
+                        if (!kind && b == a) continue;
+                        V3<T> rb;
+                        if (kind) {
+                            const V4<T> q = g.s[s];
+                            rb = mk3(q.x, q.y, q.z);
+                        } else {
+                            rb = ld3(D.pos, b);
+                        }
+                        const V3<T> rab = rb - ra;
+                        const T dist = length(rab);
+                        const T overlap = Ra + nr[b] - dist;
+                        if (!(overlap > (T)0)) continue;
+                        V3<T> tangent = mk3<T>(0, 0, 0);
+                        for (int k = 0; k < c0; ++k)
+                            if (ids[rc * kEll + k] == b) tangent = ld3(tang, rc * kEll + k);
+                        V3<T> normal = kind ? mk3<T>(0, 0, -1) : mk3<T>(1, 0, 0);
+                        if (!kind && a > b) normal = -normal;
+                        if (dist > (T)1.0e-12) normal = rab / dist;
+                        V3<T> f, disp, tq;
+                        dem_contact(normal, overlap, Ra, nr[b], va, ld3(nv, b), wa, ld3(nw, b),
+                                    tangent, K, eta, mu, p.dt, f, disp, tq);
+                        force = force + f;
+                        torque = torque + tq;
+                        if (out < kEll) {
+                            ids[rn * kEll + out] = b;
+                            st3(tang, rn * kEll + out, disp);
+                        } else {
+                            atomicOr(flags, 1);
+                        }
+                        ++out;
+                    }
+                });
+                cnt[rn] = out < kEll ? out : kEll;
+            }
+            const V3<T> fc = ((fk[0] + fk[1]) + ld3(D.drag, a)) + ld3(D.pressure_force, a);
+            st3(D.force, a, fc);
+            st3(D.torque, a, tk[0] + tk[1]);
+        }
+        __syncthreads();
+        // B. integration (Kernel_step_dem)
+        for (int a = threadIdx.x; a < n; a += kSubThreads) {
+            const V3<T> acc = ld3(D.force, a) / D.m[a];
+            const V3<T> vel = ld3(D.vel, a) + acc * p.dt;
+            const V3<T> pos = ld3(D.pos, a) + vel * p.dt;
+            st3(D.acc, a, acc);
+            st3(D.vel, a, vel);
+            st3(D.pos, a, pos);
+            st3(D.omega, a, ld3(D.omega, a) + (ld3(D.torque, a) / D.inertia[a]) * p.dt);
+            if (length(pos - ld3(pos0, a)) > skin) atomicOr(flags, 2);
+        }
+        __syncthreads();
+    }
+    // rows of the last substep -> CSR, written to both old and new
+    const int fin = n_sub & 1;
+    for (int kind = 0; kind < 2; ++kind) {
+        int *off_o = kind ? D.bnd_off_old : D.dem_off_old;
+        int *off_n = kind ? D.bnd_off_new : D.dem_off_new;
+        int *id_o = kind ? D.bnd_id_old : D.dem_id_old;
+        int *id_n = kind ? D.bnd_id_new : D.dem_id_new;
+        T *tg_o = kind ? D.bnd_tang_old : D.dem_tang_old;
+        T *tg_n = kind ? D.bnd_tang_new : D.dem_tang_new;
+        const int cap = kind ? D.cap_bnd_new : D.cap_dem_new;
+        __shared__ int total;
+        block_scan_rows<T>(cnt + row(fin, kind, 0), n, off_n, &total);
+        for (int a = threadIdx.x; a < n; a += kSubThreads) {
+            const int o = off_n[a];
+            off_o[a] = o;
+            const int r = row(fin, kind, a);
+            for (int k = 0; k < cnt[r]; ++k) {
+                if (o + k >= cap) {
+                    atomicOr(flags, 1);
+                    break;
+                }
+                const int id = ids[r * kEll + k];
+                const V3<T> tv = ld3(tang, r * kEll + k);
+                id_o[o + k] = id;
+                id_n[o + k] = id;
+                st3(tg_o, o + k, tv);
+                st3(tg_n, o + k, tv);
+            }
+        }
+        if (threadIdx.x == 0) {
+            off_o[n] = total;
+            off_n[n] = total;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+static int dem_substeps(const sphdem_params_t *p, const sphdem_dem_t *D, const sphdem_dembnd_t *W,
+                        const sphdem_grid_t *gd, const sphdem_grid_t *gw, int n_sub, double skin,
+                        void *scratch, const sphdem_work_t *w, hipStream_t st) {
+    const int n = D->n;
+    char *sp = (char *)scratch;
+    T *pos0 = (T *)sp;
+    sp += sizeof(T) * 3 * (size_t)n;
+    int *cnt = (int *)sp;
+    sp += sizeof(int) * 4 * (size_t)n;
+    int *ids = (int *)sp;
+    sp += sizeof(int) * 4 * (size_t)n * kEll;
+    T *tang = (T *)sp;
+    ProfScope prof(SPHDEM_PASS_DEM_CONTACT, st);
+    hipLaunchKernelGGL(k_dem_substeps<T>, dim3(1), dim3(kSubThreads), 0, st, prm<T>(p), view<T>(D),
+                       view<T>(W), view<T>(gd), view<T>(gw), dem_inst(gd), n_sub, (T)skin, pos0,
+                       cnt, ids, tang, w->flags);
+    return check_launch("dem_substeps");
+}
+
+size_t dem_substeps_scratch(int n, int precision) {
+    const size_t rs = precision == 8 ? 8 : 4;
+    return rs * 3 * (size_t)n + 4 * 4 * (size_t)n + 4 * 4 * (size_t)n * kEll +
+           rs * 3 * 4 * (size_t)n * kEll + 256;
+}
+
 }  // namespace sphdem
 
 using namespace sphdem;
@@ -773,6 +1012,56 @@ int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t 
     }
     DISPATCH(p->precision,
              rc = coupled_step<T>(p, P, B, D, W, gs, gb, gd, gw, step, stage, w, as_stream(stream)));
+    return rc;
+}
+
+int sphdem_surrogate_prep(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
+                          sphdem_dem_t *D, const sphdem_grid_t *gs, const sphdem_grid_t *gb,
+                          const sphdem_grid_t *gd, int step, const sphdem_work_t *w,
+                          void *stream) {
+    int rc = check_sph_pair(p, P, B, gs, gb);
+    if (rc) return rc;
+    rc = check_work(w, P->n, B->n);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(step >= 0, "step must be nonnegative");
+    if (D && D->n > 0) {
+        rc = check_dem(p, D, gd);
+        if (rc) return rc;
+        SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+        SPHDEM_REQUIRE(P->n == 0 || (P->porosity && P->pgf && P->acc_dem),
+                       "fluid coupling field is NULL");
+        SPHDEM_REQUIRE(D->flt_s && D->porosity && D->drag && D->pressure_force,
+                       "DEM coupling field is NULL");
+    }
+    DISPATCH(p->precision,
+             rc = surrogate_prep<T>(p, P, B, D, gs, gb, gd, step, w, as_stream(stream)));
+    return rc;
+}
+
+long sphdem_dem_substeps_scratch(int n, int precision) {
+    if (n < 0 || (precision != 4 && precision != 8)) return -1;
+    return (long)dem_substeps_scratch(n, precision);
+}
+
+int sphdem_dem_substeps(const sphdem_params_t *p, sphdem_dem_t *D, sphdem_dembnd_t *W,
+                        const sphdem_grid_t *gd, const sphdem_grid_t *gw, int n_sub, double skin,
+                        void *scratch, const sphdem_work_t *w, void *stream) {
+    int rc = check_dem(p, D, gd);
+    if (rc) return rc;
+    rc = check_csr(D, false);
+    if (rc) return rc;
+    rc = check_csr(D, true);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(W && gw && gw->n == W->n, "DEM-wall grid was built from a different number of points");
+    SPHDEM_REQUIRE(gd->n == D->n, "DEM grid was built from a different number of points");
+    SPHDEM_REQUIRE(n_sub >= 0 && skin >= 0.0, "invalid substep count or skin");
+    SPHDEM_REQUIRE(w && w->flags && scratch, "flags or scratch buffer is NULL");
+    SPHDEM_REQUIRE(D->contact_dem_id_old && D->tang_dem_old && D->contact_bnd_id_old &&
+                       D->tang_bnd_old,
+                   "old CSR edge buffer is NULL");
+    if (D->n == 0) return SPHDEM_OK;
+    DISPATCH(p->precision,
+             rc = dem_substeps<T>(p, D, W, gd, gw, n_sub, skin, scratch, w, as_stream(stream)));
     return rc;
 }
 

@@ -1,0 +1,841 @@
+// mpnn.hip -- the learned surrogate's message-passing network on MFMA.
+//
+// No reference counterpart: a 2-round message-passing network over the hash
+// grid's SPH neighbourhoods (radius = support) that stands in for ten WCSPH
+// substeps.  Width 64, SiLU, BF16 operands on v_mfma_f32_16x16x16_bf16 with
+// FP32 accumulation.  Parameters stay FP32 (Adam); every weight matrix is
+// packed into bf16 B-operand fragments for W (forward) and W^T (backward).
+//
+// MFMA 16x16x16 operand layout used throughout (lane l, i = l % 16,
+// g = l / 16, t = 0..3):
+//   A[m][k]: lane holds A[i][4g + t]        (short4)
+//   B[k][n]: lane holds B[4g + t][i]        (short4)
+//   D[m][n]: lane holds D[4g + t][i]        (float4)
+// A 16-row tile times a K x 64 weight is K/16 x 4 MFMAs; fragment (kb, nb)
+// of a packed weight is the short4 of lane l at ((kb * 4 + nb) * 64 + l).
+//
+// Kernels
+//   dense_fwd     Y = resid + act(X W + b) for 16-row tiles, X = [X0 | X1];
+//                 with act the FP32 output keeps the pre-activation (the
+//                 backward's input) and the bf16 output the activation
+//   dense_bwd_x   dX (+)= (dY * act'(pre)) W^T
+//   dense_bwd_w   dW += X^T (dY * act'(pre)), db += sum (dY * act'(pre))  (persistent,
+//                 per-wave accumulators, one atomic flush per wave)
+//   edge_fwd      one wave per fluid node: enumerates fluid and wall
+//                 neighbours through the hash grid (ballot-compacted edge
+//                 queue in LDS), 16 edges at a time:
+//                   z1 = silu(P_i + Q_j + e_ij C + c1)     (e C on MFMA)
+//                   z2 = silu(z1 W2 + c2)                  (MFMA)
+//                   s_i += z2 rows;  s_i *= 1/64 at the end
+//   edge_bwd      the same enumeration with the chunk recomputed; dW2, dC,
+//                 dc1, dc2, dQ_wall accumulate per wave, dP_i per node, dQ_j
+//                 by atomics (j is a neighbour row)
+//   features / targets / loss / apply / adam: per-node elementwise work
+#include "common.hpp"
+
+#include <cmath>
+
+namespace sphdem {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kF = 64;          // hidden width
+constexpr int kEdgeFeat = 16;   // edge features (9 used)
+constexpr float kSumScale = 1.0f / 64.0f;
+constexpr float kUScale = 1.0f;     // velocity scale of the features, m/s
+constexpr float kAScale = 9.81f;    // acceleration scale, m/s^2
+
+__device__ __forceinline__ unsigned short f2bf(float x) {
+    const unsigned int b = __float_as_uint(x);
+    return (unsigned short)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) {
+    return __uint_as_float(((unsigned int)h) << 16);
+}
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)f2bf(a);
+    r[1] = (short)f2bf(b);
+    r[2] = (short)f2bf(c);
+    r[3] = (short)f2bf(d);
+    return r;
+}
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
+
+// SiLU x / (1 + e^-x) and its derivative s (1 + x (1 - s)), s = 1 / (1 + e^-x).
+__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
+__device__ __forceinline__ float dsilu(float x) {
+    const float s = 1.0f / (1.0f + __expf(-x));
+    return s * (1.0f + x * (1.0f - s));
+}
+
+// ---------------------------------------------------------------- packing
+// Bp: fragments of W (K x 64) as a B operand; BTp: of W^T (64 x K).
+__global__ void k_pack(const float *W, int K, bf16x4 *Bp, bf16x4 *BTp) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nfr = (K / 16) * 4 * 64;
+    if (idx >= nfr) return;
+    const int l = idx & 63, i = l & 15, g = l >> 4, fr = idx >> 6;
+    {   // W: fragment fr = kb * 4 + nb
+        const int kb = fr >> 2, nb = fr & 3;
+        const float *w = W + (kb * 16 + 4 * g) * kF + nb * 16 + i;
+        Bp[idx] = pack4(w[0], w[kF], w[2 * kF], w[3 * kF]);
+    }
+    {   // W^T: K' = 64 (4 blocks), N' = K (K/16 blocks); fragment fr = kb' * (K/16) + nb'
+        const int nbn = K / 16, kb = fr / nbn, nb = fr % nbn;
+        const float *w = W + (nb * 16 + i) * kF + kb * 16 + 4 * g;
+        BTp[idx] = pack4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// ---------------------------------------------------------------- dense layers
+__device__ __forceinline__ bf16x4 load_a_row(const unsigned short *X0, int K0,
+                                             const unsigned short *X1, int K1, int row, int rows,
+                                             int k) {
+    if (row >= rows) return bf16x4{0, 0, 0, 0};
+    const unsigned short *p = k < K0 ? X0 + (size_t)row * K0 + k : X1 + (size_t)row * K1 + (k - K0);
+    const uint2 v = *reinterpret_cast<const uint2 *>(p);
+    bf16x4 r;
+    r[0] = (short)(v.x & 0xffff);
+    r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xffff);
+    r[3] = (short)(v.y >> 16);
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_dense_fwd(const unsigned short *X0, int K0,
+                                                   const unsigned short *X1, int K1, int rows,
+                                                   const bf16x4 *Bp, const float *b, int act,
+                                                   const float *resid, float *Yf,
+                                                   unsigned short *Yb) {
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const int r0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (r0 >= rows) return;
+    const int K = K0 + K1;
+    f32x4 acc[4] = {zero4(), zero4(), zero4(), zero4()};
+    for (int kb = 0; kb < K / 16; ++kb) {
+        const bf16x4 a = load_a_row(X0, K0, X1, K1, r0 + i, rows, kb * 16 + 4 * g);
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) acc[nb] = mfma(a, Bp[(kb * 4 + nb) * 64 + lane], acc[nb]);
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        const int col = nb * 16 + i;
+        const float bias = b ? b[col] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int row = r0 + 4 * g + t;
+            if (row >= rows) continue;
+            float y = acc[nb][t] + bias;
+            if (act) {
+                if (Yf) Yf[(size_t)row * kF + col] = y;
+                if (Yb) Yb[(size_t)row * kF + col] = f2bf(silu(y));
+                continue;
+            }
+            if (resid) y = resid[(size_t)row * kF + col] + y;
+            if (Yf) Yf[(size_t)row * kF + col] = y;
+            if (Yb) Yb[(size_t)row * kF + col] = f2bf(y);
+        }
+    }
+}
+
+// dZ = dY * act'(pre) where the pre-activation is given: the gradient at
+// the pre-activation.
+__device__ __forceinline__ float dz_at(const float *dY, const float *pre, int row, int col) {
+    const float d = dY[(size_t)row * kF + col];
+    return pre ? d * dsilu(pre[(size_t)row * kF + col]) : d;
+}
+
+__global__ __launch_bounds__(256) void k_dense_bwd_x(const float *dY, const float *Ym, int rows,
+                                                     const bf16x4 *BTp, int K, float *dX0, int K0,
+                                                     float *dX1, int K1, int accumulate) {
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const int r0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16;
+    if (r0 >= rows) return;
+    const int nbn = K / 16;
+    const int ra = r0 + i;
+    bf16x4 a[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        if (ra < rows)
+            for (int t = 0; t < 4; ++t) v[t] = dz_at(dY, Ym, ra, kb * 16 + 4 * g + t);
+        a[kb] = pack4(v[0], v[1], v[2], v[3]);
+    }
+    for (int nb = 0; nb < nbn; ++nb) {
+        f32x4 acc = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) acc = mfma(a[kb], BTp[(kb * nbn + nb) * 64 + lane], acc);
+        const int f = nb * 16 + i;
+        float *dst = f < K0 ? dX0 : dX1;
+        const int ld = f < K0 ? K0 : K1, fc = f < K0 ? f : f - K0;
+        for (int t = 0; t < 4; ++t) {
+            const int row = r0 + 4 * g + t;
+            if (row >= rows) continue;
+            float *p = dst + (size_t)row * ld + fc;
+            *p = accumulate ? *p + acc[t] : acc[t];
+        }
+    }
+}
+
+template <int KB>
+__global__ __launch_bounds__(256) void k_dense_bwd_w(const float *dY, const float *Ym,
+                                                     const unsigned short *X0, int K0,
+                                                     const unsigned short *X1, int K1, int rows,
+                                                     float *gW, float *gb) {
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    f32x4 acc[KB][4];
+    for (int mb = 0; mb < KB; ++mb)
+        for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = zero4();
+    float db[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int r0 = wave * 16; r0 < rows; r0 += nwaves * 16) {
+        bf16x4 bz[4];
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            float v[4];
+            for (int t = 0; t < 4; ++t) {
+                const int row = r0 + 4 * g + t;
+                v[t] = row < rows ? dz_at(dY, Ym, row, nb * 16 + i) : 0.f;
+                db[nb] += v[t];
+            }
+            bz[nb] = pack4(v[0], v[1], v[2], v[3]);
+        }
+#pragma unroll
+        for (int mb = 0; mb < KB; ++mb) {
+            const int f = mb * 16 + i;
+            const unsigned short *src = f < K0 ? X0 + f : X1 + (f - K0);
+            const int ld = f < K0 ? K0 : K1;
+            bf16x4 a;
+            for (int t = 0; t < 4; ++t) {
+                const int row = r0 + 4 * g + t;
+                a[t] = row < rows ? (short)src[(size_t)row * ld] : (short)0;
+            }
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = mfma(a, bz[nb], acc[mb][nb]);
+        }
+    }
+    for (int mb = 0; mb < KB; ++mb)
+        for (int nb = 0; nb < 4; ++nb)
+            for (int t = 0; t < 4; ++t)
+                atomicAdd(&gW[(mb * 16 + 4 * g + t) * kF + nb * 16 + i], acc[mb][nb][t]);
+    if (gb) {
+        for (int nb = 0; nb < 4; ++nb) {
+            float v = db[nb];
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+            if (g == 0) atomicAdd(&gb[nb * 16 + i], v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- edge passes
+struct EdgeArgs {
+    GridV<float> gs, gb;
+    const float4 *fvel;        // fluid velocity, sorted slots (x, y, z, -)
+    const float4 *wvel;        // wall velocity, sorted slots, or null (at rest)
+    float h, inv_h, support;
+    const float *P;            // [N x 64] h_i Wa
+    const unsigned short *Q;   // [N x 64] bf16 h_j Wb
+    const unsigned short *Qw;  // [64] bf16 hw Wb
+    const bf16x4 *Cp;          // C: 16 x 64, 4 fragments
+    const float *c1;
+    const bf16x4 *W2p, *W2Tp;  // W2 and W2^T fragments
+    const float *c2;
+};
+
+// Per-wave LDS: the edge queue and the chunk staging buffers.
+struct EdgeLds {
+    int slot[96];
+    int kind[96];
+    unsigned short z1[16][kF + 4];
+    unsigned short d2[16][kF + 4];
+    unsigned short e[16][kEdgeFeat + 4];
+};
+
+// Features of edge (me -> q): (dx, dy, dz)/h, r/h, dv/U, wall flag, W h^3.
+__device__ __forceinline__ float edge_feature(int f, float dx, float dy, float dz, float r,
+                                              float dvx, float dvy, float dvz, float wall,
+                                              float inv_h) {
+    switch (f) {
+    case 0: return dx * inv_h;
+    case 1: return dy * inv_h;
+    case 2: return dz * inv_h;
+    case 3: return r * inv_h;
+    case 4: return dvx * (1.0f / kUScale);
+    case 5: return dvy * (1.0f / kUScale);
+    case 6: return dvz * (1.0f / kUScale);
+    case 7: return wall;
+    case 8: {
+        const float q = r * inv_h, u = 1.0f - 0.5f * q;
+        return q < 2.0f ? (float)(21.0 / (16.0 * SPHDEM_PI)) * u * u * u * u * (1.0f + 2.0f * q)
+                        : 0.0f;
+    }
+    default: return 0.0f;
+    }
+}
+
+// Recompute one chunk of up to 16 queued edges of node k: pre1 and z1 in D
+// layout (rows 4g+t = edges), z1 staged in LDS, pre2 in D layout.
+struct Chunk {
+    f32x4 pre1[4], pre2[4];
+    int nvalid;
+};
+
+__device__ __forceinline__ void chunk_forward(const EdgeArgs &A, EdgeLds &L, const V4<float> &me,
+                                              float4 vme, const float *Pk, int nvalid, int lane,
+                                              Chunk &c, bool keep_e) {
+    const int i = lane & 15, g = lane >> 4;
+    // edge features of row i, columns 4g..4g+3 (A operand of e C)
+    float fe[4] = {0.f, 0.f, 0.f, 0.f};
+    if (i < nvalid) {
+        const int s = L.slot[i], kind = L.kind[i];
+        const V4<float> q = kind ? A.gb.s[s] : A.gs.s[s];
+        float4 vq = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (kind == 0) vq = A.fvel[s];
+        else if (A.wvel) vq = A.wvel[s];
+        const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+        for (int t = 0; t < 4; ++t)
+            fe[t] = edge_feature(4 * g + t, dx, dy, dz, r, vq.x - vme.x, vq.y - vme.y,
+                                 vq.z - vme.z, (float)kind, A.inv_h);
+    }
+    const bf16x4 ea = pack4(fe[0], fe[1], fe[2], fe[3]);
+    if (keep_e)
+        for (int t = 0; t < 4; ++t) L.e[i][4 * g + t] = (unsigned short)ea[t];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) c.pre1[nb] = mfma(ea, A.Cp[nb * 64 + lane], zero4());
+    // + P_i + Q_j + c1, silu, bf16 -> LDS (row 4g+t, col nb*16+i); rows past
+    // nvalid are padding: z1 = 0 there
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int row = 4 * g + t;
+        const bool ok = row < nvalid;
+        const int s = ok ? L.slot[row] : 0, kind = ok ? L.kind[row] : 1;
+        const unsigned short *qrow = kind ? A.Qw : A.Q + (size_t)s * kF;
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int col = nb * 16 + i;
+            float v = c.pre1[nb][t] + Pk[nb] + bf2f(qrow[col]) + A.c1[col];
+            c.pre1[nb][t] = v;
+            const float z = ok ? silu(v) : 0.f;
+            L.z1[row][col] = f2bf(z);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    bf16x4 za[4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+        for (int t = 0; t < 4; ++t) za[kb][t] = (short)L.z1[i][kb * 16 + 4 * g + t];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        f32x4 acc = zero4();
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) acc = mfma(za[kb], A.W2p[(kb * 4 + nb) * 64 + lane], acc);
+        const float cb = A.c2[nb * 16 + i];
+        for (int t = 0; t < 4; ++t) acc[t] = acc[t] + cb;
+        c.pre2[nb] = acc;
+    }
+    c.nvalid = nvalid;
+}
+
+// Enumerate the node's neighbours (fluid grid, then wall grid) in query
+// order and hand them to `proc(nvalid)` 16 at a time through L.slot/kind.
+template <typename Proc>
+__device__ __forceinline__ void for_edges(const EdgeArgs &A, EdgeLds &L, const V4<float> &me,
+                                          int lane, Proc &&proc) {
+    const float ss = A.support * A.support;
+    int count = 0;
+    for (int kind = 0; kind < 2; ++kind) {
+        const GridV<float> &g = kind ? A.gb : A.gs;
+        query_runs(g, me.x, me.y, me.z, A.support, [&](int lo, int hi) {
+            for (int base = lo; base < hi; base += 64) {
+                const int s = base + lane;
+                bool ok = false;
+                if (s < hi) {
+                    const V4<float> q = g.s[s];
+                    const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+                    const float td = dx * dx + dy * dy + dz * dz;
+                    ok = td > 1.0e-12f && td < ss;
+                }
+                const unsigned long long m = __ballot(ok);
+                if (ok) {
+                    const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+                    L.slot[pos] = s;
+                    L.kind[pos] = kind;
+                }
+                count += __popcll(m);
+                __builtin_amdgcn_wave_barrier();
+                int done = 0;
+                while (count - done >= 16) {
+                    if (done) {   // move the next 16 to the front
+                        const int a = lane < 16 ? L.slot[done + lane] : 0;
+                        const int b = lane < 16 ? L.kind[done + lane] : 0;
+                        __builtin_amdgcn_wave_barrier();
+                        if (lane < 16) { L.slot[lane] = a; L.kind[lane] = b; }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    proc(16);
+                    done += 16;
+                }
+                if (done) {
+                    const int left = count - done;
+                    const int a = lane < left ? L.slot[done + lane] : 0;
+                    const int b = lane < left ? L.kind[done + lane] : 0;
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane < left) { L.slot[lane] = a; L.kind[lane] = b; }
+                    count = left;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        });
+    }
+    if (count > 0) proc(count);
+}
+
+__global__ __launch_bounds__(256) void k_edge_fwd(EdgeArgs A, float *s_out,
+                                                  unsigned short *s_bf) {
+    __shared__ EdgeLds lds[4];
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= A.gs.n) return;
+    EdgeLds &L = lds[threadIdx.x >> 6];
+    const V4<float> me = A.gs.s[k];
+    const float4 vme = A.fvel[k];
+    EdgeArgs E = A;
+    const int ik = inst_of(A.gs, k);
+    E.gs = at_inst(A.gs, ik);
+    E.gb = at_inst(A.gb, ik);
+    float Pk[4];
+    for (int nb = 0; nb < 4; ++nb) Pk[nb] = A.P[(size_t)k * kF + nb * 16 + i];
+    float sacc[4] = {0.f, 0.f, 0.f, 0.f};
+    for_edges(E, L, me, lane, [&](int nvalid) {
+        Chunk c;
+        chunk_forward(E, L, me, vme, Pk, nvalid, lane, c, false);
+        for (int nb = 0; nb < 4; ++nb)
+            for (int t = 0; t < 4; ++t)
+                if (4 * g + t < nvalid) sacc[nb] += silu(c.pre2[nb][t]);
+        __builtin_amdgcn_wave_barrier();
+    });
+    for (int nb = 0; nb < 4; ++nb) {
+        float v = sacc[nb];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (g == 0) {
+            const float y = v * kSumScale;
+            s_out[(size_t)k * kF + nb * 16 + i] = y;
+            if (s_bf) s_bf[(size_t)k * kF + nb * 16 + i] = f2bf(y);
+        }
+    }
+}
+
+struct EdgeGrads {
+    const float *ds;   // [N x 64] gradient at s (after the 1/64 scale)
+    float *dP;         // [N x 64] written
+    float *dQ;         // [N x 64] accumulated (atomics)
+    float *gC, *gc1, *gW2, *gc2, *gQw;
+};
+
+__global__ __launch_bounds__(256) void k_edge_bwd(EdgeArgs A, EdgeGrads G) {
+    __shared__ EdgeLds lds[4];
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    EdgeLds &L = lds[threadIdx.x >> 6];
+    f32x4 gw2[4][4], gc[4];
+    for (int a = 0; a < 4; ++a) {
+        gc[a] = zero4();
+        for (int b = 0; b < 4; ++b) gw2[a][b] = zero4();
+    }
+    float gc1[4] = {0.f, 0.f, 0.f, 0.f}, gc2[4] = {0.f, 0.f, 0.f, 0.f};
+    float gqw[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = wave; k < A.gs.n; k += nwaves) {
+        const V4<float> me = A.gs.s[k];
+        const float4 vme = A.fvel[k];
+        EdgeArgs E = A;
+        const int ik = inst_of(A.gs, k);
+        E.gs = at_inst(A.gs, ik);
+        E.gb = at_inst(A.gb, ik);
+        float Pk[4], dsk[4], pacc[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int nb = 0; nb < 4; ++nb) {
+            Pk[nb] = A.P[(size_t)k * kF + nb * 16 + i];
+            dsk[nb] = G.ds[(size_t)k * kF + nb * 16 + i] * kSumScale;
+        }
+        for_edges(E, L, me, lane, [&](int nvalid) {
+            Chunk c;
+            chunk_forward(E, L, me, vme, Pk, nvalid, lane, c, true);
+            // dpre2 (D layout) -> bias, dW2 = z1^T dpre2, LDS for the transpose
+            bf16x4 d2b[4];
+            for (int nb = 0; nb < 4; ++nb) {
+                float v[4];
+                for (int t = 0; t < 4; ++t) {
+                    v[t] = 4 * g + t < nvalid ? dsk[nb] * dsilu(c.pre2[nb][t]) : 0.f;
+                    gc2[nb] += v[t];
+                }
+                d2b[nb] = pack4(v[0], v[1], v[2], v[3]);
+                for (int t = 0; t < 4; ++t) L.d2[4 * g + t][nb * 16 + i] = (unsigned short)d2b[nb][t];
+            }
+            bf16x4 z1t[4];
+            for (int mb = 0; mb < 4; ++mb)
+                for (int t = 0; t < 4; ++t) z1t[mb][t] = (short)L.z1[4 * g + t][mb * 16 + i];
+            for (int mb = 0; mb < 4; ++mb)
+                for (int nb = 0; nb < 4; ++nb) gw2[mb][nb] = mfma(z1t[mb], d2b[nb], gw2[mb][nb]);
+            __builtin_amdgcn_wave_barrier();
+            // dz1 = dpre2 W2^T: A operand from LDS (rows = edges i)
+            bf16x4 da[4];
+            for (int kb = 0; kb < 4; ++kb)
+                for (int t = 0; t < 4; ++t) da[kb][t] = (short)L.d2[i][kb * 16 + 4 * g + t];
+            bf16x4 d1b[4];
+            float d1[4][4];
+            for (int nb = 0; nb < 4; ++nb) {
+                f32x4 acc = zero4();
+                for (int kb = 0; kb < 4; ++kb)
+                    acc = mfma(da[kb], A.W2Tp[(kb * 4 + nb) * 64 + lane], acc);
+                for (int t = 0; t < 4; ++t) {
+                    const float v = 4 * g + t < nvalid ? acc[t] * dsilu(c.pre1[nb][t]) : 0.f;
+                    d1[nb][t] = v;
+                    gc1[nb] += v;
+                    pacc[nb] += v;
+                }
+                d1b[nb] = pack4(d1[nb][0], d1[nb][1], d1[nb][2], d1[nb][3]);
+            }
+            // dC = e^T dpre1 (A operand: e[4g+t][i] from LDS)
+            bf16x4 et;
+            for (int t = 0; t < 4; ++t) et[t] = (short)L.e[4 * g + t][i];
+            for (int nb = 0; nb < 4; ++nb) gc[nb] = mfma(et, d1b[nb], gc[nb]);
+            // dQ_j (fluid rows) / dQ_wall
+            for (int t = 0; t < 4; ++t) {
+                const int row = 4 * g + t;
+                if (row >= nvalid) continue;
+                const int s = L.slot[row];
+                if (L.kind[row] == 0) {
+                    for (int nb = 0; nb < 4; ++nb)
+                        atomicAdd(&G.dQ[(size_t)s * kF + nb * 16 + i], d1[nb][t]);
+                } else {
+                    for (int nb = 0; nb < 4; ++nb) gqw[nb] += d1[nb][t];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        });
+        for (int nb = 0; nb < 4; ++nb) {
+            float v = pacc[nb];
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+            if (g == 0) G.dP[(size_t)k * kF + nb * 16 + i] = v;
+        }
+    }
+    for (int mb = 0; mb < 4; ++mb)
+        for (int nb = 0; nb < 4; ++nb)
+            for (int t = 0; t < 4; ++t)
+                atomicAdd(&G.gW2[(mb * 16 + 4 * g + t) * kF + nb * 16 + i], gw2[mb][nb][t]);
+    for (int nb = 0; nb < 4; ++nb)
+        for (int t = 0; t < 4; ++t)
+            atomicAdd(&G.gC[(4 * g + t) * kF + nb * 16 + i], gc[nb][t]);
+    for (int nb = 0; nb < 4; ++nb) {
+        float a = gc1[nb], b = gc2[nb], q = gqw[nb];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        b += __shfl_xor(b, 16, 64);
+        b += __shfl_xor(b, 32, 64);
+        q += __shfl_xor(q, 16, 64);
+        q += __shfl_xor(q, 32, 64);
+        if (g == 0) {
+            atomicAdd(&G.gc1[nb * 16 + i], a);
+            atomicAdd(&G.gc2[nb * 16 + i], b);
+            atomicAdd(&G.gQw[nb * 16 + i], q);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- per node
+// Node features in sorted order (bf16, 16 wide) and the sorted velocity.
+__global__ void k_features(GridV<float> gs, const float *vel, const float *acc, const float *rho,
+                           float rho0, unsigned short *nf, float4 *fvel) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const int id = gs.ids[k];
+    const float v[3] = {vel[3 * id], vel[3 * id + 1], vel[3 * id + 2]};
+    const float a[3] = {acc[3 * id], acc[3 * id + 1], acc[3 * id + 2]};
+    unsigned short *o = nf + (size_t)k * 16;
+    for (int c = 0; c < 3; ++c) {
+        o[c] = f2bf(v[c] * (1.0f / kUScale));
+        o[3 + c] = f2bf(a[c] * (1.0f / kAScale));
+    }
+    o[6] = f2bf(100.0f * (rho[id] / rho0 - 1.0f));
+    for (int c = 7; c < 16; ++c) o[c] = 0;
+    fvel[k] = make_float4(v[0], v[1], v[2], 0.f);
+}
+
+// Training targets in sorted order: the ten-substep change beyond one
+// explicit step of length T from (x0, v0, a0).
+__global__ void k_targets(GridV<float> gs, const float *x0, const float *v0, const float *a0,
+                          const float *x1, const float *v1, float T, float *target) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const int id = gs.ids[k];
+    for (int c = 0; c < 3; ++c) {
+        const float a = a0[3 * id + c], v = v0[3 * id + c];
+        target[(size_t)k * 6 + c] = (v1[3 * id + c] - v - T * a) / (kAScale * T);
+        target[(size_t)k * 6 + 3 + c] =
+            (x1[3 * id + c] - x0[3 * id + c] - T * v - 0.5f * T * T * a) / (0.5f * kAScale * T * T);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_loss(const float *out, const float *target, int n,
+                                              double *loss, float *dout) {
+    double acc = 0.0;
+    const float scale = 2.0f / (6.0f * (float)n);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        for (int c = 0; c < kF; ++c) {
+            float d = 0.f;
+            if (c < 6) {
+                d = out[(size_t)k * kF + c] - target[(size_t)k * 6 + c];
+                acc += (double)d * (double)d;
+            }
+            dout[(size_t)k * kF + c] = c < 6 ? scale * d : 0.f;
+        }
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc / (6.0 * (double)n));
+}
+
+// One surrogate call: dv = T a + A T out[0:3], dx = T v + T^2/2 a + A T^2/2 out[3:6].
+__global__ void k_apply(GridV<float> gs, const float *out, float T, float *pos, float *vel,
+                        const float *acc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const int id = gs.ids[k];
+    for (int c = 0; c < 3; ++c) {
+        const float a = acc[3 * id + c], v = vel[3 * id + c];
+        const float dv = T * a + kAScale * T * out[(size_t)k * kF + c];
+        const float dx = T * v + 0.5f * T * T * a + 0.5f * kAScale * T * T * out[(size_t)k * kF + 3 + c];
+        vel[3 * id + c] = v + dv;
+        pos[3 * id + c] = pos[3 * id + c] + dx;
+    }
+}
+
+__global__ void k_to_bf16(const float *x, int n, unsigned short *y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = f2bf(x[k]);
+}
+
+__global__ void k_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+}
+
+}  // namespace sphdem
+
+using namespace sphdem;
+
+static int mpnn_grid_ok(const sphdem_grid_t *g) {
+    SPHDEM_REQUIRE(g && g->precision == 4, "the surrogate runs on FP32 grids");
+    SPHDEM_REQUIRE(g->n == 0 || (g->point_ids && g->sorted), "grid is not built");
+    return SPHDEM_OK;
+}
+
+static EdgeArgs edge_args(const sphdem_params_t *p, const sphdem_grid_t *gs,
+                          const sphdem_grid_t *gb, const void *fvel, const void *wvel,
+                          const float *P, const void *Q, const void *Qw, const void *Cp,
+                          const float *c1, const void *W2p, const void *W2Tp, const float *c2) {
+    EdgeArgs A;
+    A.gs = view<float>(gs);
+    A.gb = view<float>(gb);
+    A.fvel = (const float4 *)fvel;
+    A.wvel = (const float4 *)wvel;
+    A.h = (float)p->h;
+    A.inv_h = 1.0f / (float)p->h;
+    A.support = (float)p->support;
+    A.P = P;
+    A.Q = (const unsigned short *)Q;
+    A.Qw = (const unsigned short *)Qw;
+    A.Cp = (const bf16x4 *)Cp;
+    A.c1 = c1;
+    A.W2p = (const bf16x4 *)W2p;
+    A.W2Tp = (const bf16x4 *)W2Tp;
+    A.c2 = c2;
+    return A;
+}
+
+extern "C" {
+
+int sphdem_mpnn_pack(const float *W, int K, void *Bp, void *BTp, void *stream) {
+    SPHDEM_REQUIRE(W && Bp && BTp && K > 0 && K % 16 == 0, "invalid weight packing arguments");
+    const int n = (K / 16) * 4 * 64;
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), W, K,
+                       (bf16x4 *)Bp, (bf16x4 *)BTp);
+    return check_launch("mpnn_pack");
+}
+
+int sphdem_mpnn_dense_fwd(const void *X0, int K0, const void *X1, int K1, int rows,
+                          const void *Bp, const float *b, int act, const float *resid,
+                          float *Yf, void *Yb, void *stream) {
+    SPHDEM_REQUIRE(rows >= 0 && K0 > 0 && K0 % 16 == 0 && K1 >= 0 && K1 % 16 == 0,
+                   "dense layer widths must be positive multiples of 16");
+    SPHDEM_REQUIRE(X0 && Bp && (K1 == 0 || X1) && (Yf || Yb), "NULL dense layer buffer");
+    if (rows == 0) return SPHDEM_OK;
+    const int tiles = blocks_for(rows, 16);
+    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks_for(tiles, 4)), dim3(256), 0, as_stream(stream),
+                       (const unsigned short *)X0, K0, (const unsigned short *)X1, K1, rows,
+                       (const bf16x4 *)Bp, b, act, resid, Yf, (unsigned short *)Yb);
+    return check_launch("mpnn_dense_fwd");
+}
+
+int sphdem_mpnn_dense_bwd_x(const float *dY, const float *Ymask, int rows, const void *BTp,
+                            float *dX0, int K0, float *dX1, int K1, int accumulate, void *stream) {
+    SPHDEM_REQUIRE(rows >= 0 && K0 > 0 && K0 % 16 == 0 && K1 >= 0 && K1 % 16 == 0,
+                   "dense layer widths must be positive multiples of 16");
+    SPHDEM_REQUIRE(dY && BTp && dX0 && (K1 == 0 || dX1), "NULL dense layer buffer");
+    if (rows == 0) return SPHDEM_OK;
+    const int tiles = blocks_for(rows, 16);
+    hipLaunchKernelGGL(k_dense_bwd_x, dim3(blocks_for(tiles, 4)), dim3(256), 0, as_stream(stream),
+                       dY, Ymask, rows, (const bf16x4 *)BTp, K0 + K1, dX0, K0, dX1, K1, accumulate);
+    return check_launch("mpnn_dense_bwd_x");
+}
+
+int sphdem_mpnn_dense_bwd_w(const float *dY, const float *Ymask, const void *X0, int K0,
+                            const void *X1, int K1, int rows, float *gW, float *gb, void *stream) {
+    SPHDEM_REQUIRE(rows >= 0 && K0 > 0 && K0 % 16 == 0 && K1 >= 0 && K1 % 16 == 0,
+                   "dense layer widths must be positive multiples of 16");
+    SPHDEM_REQUIRE(dY && X0 && (K1 == 0 || X1) && gW, "NULL dense layer buffer");
+    const int K = K0 + K1;
+    SPHDEM_REQUIRE(K == 16 || K == 64 || K == 128, "dense layer input width must be 16, 64 or 128");
+    if (rows == 0) return SPHDEM_OK;
+    const int tiles = blocks_for(rows, 16);
+    const int blocks = tiles < 4 * 512 ? blocks_for(tiles, 4) : 512;
+    hipStream_t st = as_stream(stream);
+    const unsigned short *x0 = (const unsigned short *)X0, *x1 = (const unsigned short *)X1;
+    if (K == 16)
+        hipLaunchKernelGGL(k_dense_bwd_w<1>, dim3(blocks), dim3(256), 0, st, dY, Ymask, x0, K0, x1,
+                           K1, rows, gW, gb);
+    else if (K == 64)
+        hipLaunchKernelGGL(k_dense_bwd_w<4>, dim3(blocks), dim3(256), 0, st, dY, Ymask, x0, K0, x1,
+                           K1, rows, gW, gb);
+    else
+        hipLaunchKernelGGL(k_dense_bwd_w<8>, dim3(blocks), dim3(256), 0, st, dY, Ymask, x0, K0, x1,
+                           K1, rows, gW, gb);
+    return check_launch("mpnn_dense_bwd_w");
+}
+
+int sphdem_mpnn_edge_fwd(const sphdem_params_t *p, const sphdem_grid_t *gs,
+                         const sphdem_grid_t *gb, const void *fvel, const void *wvel,
+                         const float *P, const void *Q, const void *Qw, const void *Cp,
+                         const float *c1, const void *W2p, const float *c2, float *s,
+                         void *s_bf, void *stream) {
+    SPHDEM_REQUIRE(p && p->precision == 4, "the surrogate runs in FP32");
+    int rc = mpnn_grid_ok(gs);
+    if (rc) return rc;
+    rc = mpnn_grid_ok(gb);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(fvel && P && Q && Qw && Cp && c1 && W2p && c2 && s, "NULL edge pass buffer");
+    if (gs->n == 0) return SPHDEM_OK;
+    const EdgeArgs A = edge_args(p, gs, gb, fvel, wvel, P, Q, Qw, Cp, c1, W2p, nullptr, c2);
+    hipLaunchKernelGGL(k_edge_fwd, dim3(blocks_for(gs->n, 4)), dim3(256), 0, as_stream(stream), A,
+                       s, (unsigned short *)s_bf);
+    return check_launch("mpnn_edge_fwd");
+}
+
+int sphdem_mpnn_edge_bwd(const sphdem_params_t *p, const sphdem_grid_t *gs,
+                         const sphdem_grid_t *gb, const void *fvel, const void *wvel,
+                         const float *P, const void *Q, const void *Qw, const void *Cp,
+                         const float *c1, const void *W2p, const void *W2Tp, const float *c2,
+                         const float *ds, float *dP, float *dQ, float *gC, float *gc1,
+                         float *gW2, float *gc2, float *gQw, void *stream) {
+    SPHDEM_REQUIRE(p && p->precision == 4, "the surrogate runs in FP32");
+    int rc = mpnn_grid_ok(gs);
+    if (rc) return rc;
+    rc = mpnn_grid_ok(gb);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(fvel && P && Q && Qw && Cp && c1 && W2p && W2Tp && c2 && ds && dP && dQ &&
+                       gC && gc1 && gW2 && gc2 && gQw,
+                   "NULL edge pass buffer");
+    if (gs->n == 0) return SPHDEM_OK;
+    const EdgeArgs A = edge_args(p, gs, gb, fvel, wvel, P, Q, Qw, Cp, c1, W2p, W2Tp, c2);
+    EdgeGrads G{ds, dP, dQ, gC, gc1, gW2, gc2, gQw};
+    const int blocks = blocks_for(gs->n, 4) < 1024 ? blocks_for(gs->n, 4) : 1024;
+    hipLaunchKernelGGL(k_edge_bwd, dim3(blocks), dim3(256), 0, as_stream(stream), A, G);
+    return check_launch("mpnn_edge_bwd");
+}
+
+int sphdem_mpnn_features(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_grid_t *gs,
+                         void *nfeat, void *fvel, void *stream) {
+    SPHDEM_REQUIRE(p && P && p->precision == 4, "the surrogate runs in FP32");
+    int rc = mpnn_grid_ok(gs);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(gs->n == P->n, "fluid grid was built from a different number of points");
+    SPHDEM_REQUIRE(P->n == 0 || (P->vel && P->acc && P->rho && nfeat && fvel), "NULL buffer");
+    if (P->n == 0) return SPHDEM_OK;
+    hipLaunchKernelGGL(k_features, dim3(blocks_for(P->n, 256)), dim3(256), 0, as_stream(stream),
+                       view<float>(gs), (const float *)P->vel, (const float *)P->acc,
+                       (const float *)P->rho, (float)p->rho0, (unsigned short *)nfeat,
+                       (float4 *)fvel);
+    return check_launch("mpnn_features");
+}
+
+int sphdem_mpnn_targets(const sphdem_grid_t *gs, const float *x0, const float *v0,
+                        const float *a0, const float *x1, const float *v1, double T,
+                        float *target, void *stream) {
+    int rc = mpnn_grid_ok(gs);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(gs->n == 0 || (x0 && v0 && a0 && x1 && v1 && target), "NULL buffer");
+    if (gs->n == 0) return SPHDEM_OK;
+    hipLaunchKernelGGL(k_targets, dim3(blocks_for(gs->n, 256)), dim3(256), 0, as_stream(stream),
+                       view<float>(gs), x0, v0, a0, x1, v1, (float)T, target);
+    return check_launch("mpnn_targets");
+}
+
+int sphdem_mpnn_loss(const float *out, const float *target, int n, double *loss, float *dout,
+                     void *stream) {
+    SPHDEM_REQUIRE(n > 0 && out && target && loss && dout, "invalid loss arguments");
+    const int blocks = blocks_for(n, 256) < 1024 ? blocks_for(n, 256) : 1024;
+    hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, as_stream(stream), out, target, n, loss,
+                       dout);
+    return check_launch("mpnn_loss");
+}
+
+int sphdem_mpnn_apply(const sphdem_params_t *p, sphdem_sph_t *P, const sphdem_grid_t *gs,
+                      const float *out, double T, void *stream) {
+    SPHDEM_REQUIRE(p && P && p->precision == 4, "the surrogate runs in FP32");
+    int rc = mpnn_grid_ok(gs);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(gs->n == P->n, "fluid grid was built from a different number of points");
+    SPHDEM_REQUIRE(P->n == 0 || (P->pos && P->vel && P->acc && out), "NULL buffer");
+    if (P->n == 0) return SPHDEM_OK;
+    hipLaunchKernelGGL(k_apply, dim3(blocks_for(P->n, 256)), dim3(256), 0, as_stream(stream),
+                       view<float>(gs), out, (float)T, (float *)P->pos, (float *)P->vel,
+                       (const float *)P->acc);
+    return check_launch("mpnn_apply");
+}
+
+int sphdem_mpnn_to_bf16(const float *x, int n, void *y, void *stream) {
+    SPHDEM_REQUIRE(n >= 0 && (n == 0 || (x && y)), "invalid conversion buffers");
+    if (n == 0) return SPHDEM_OK;
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), x, n,
+                       (unsigned short *)y);
+    return check_launch("mpnn_to_bf16");
+}
+
+int sphdem_adam(float *param, const float *grad, float *m, float *v, int n, double lr,
+                double beta1, double beta2, double eps, int step, void *stream) {
+    SPHDEM_REQUIRE(n >= 0 && step >= 1 && (n == 0 || (param && grad && m && v)),
+                   "invalid Adam arguments");
+    if (n == 0) return SPHDEM_OK;
+    const float c1 = (float)(1.0 - std::pow(beta1, step)), c2 = (float)(1.0 - std::pow(beta2, step));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), param,
+                       grad, m, v, n, (float)lr, (float)beta1, (float)beta2, (float)eps, c1, c2);
+    return check_launch("adam");
+}
+
+}  // extern "C"
