@@ -332,15 +332,19 @@ int sphdem_mpnn_edge_bwd(const sphdem_params_t *p, const sphdem_grid_t *gs,
  * A = 9.81 m/s^2) and the sorted velocity [N x 4]. */
 int sphdem_mpnn_features(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_grid_t *gs,
                          void *nfeat, void *fvel, void *stream);
-/* Training target [N x 6] (sorted): ((v1 - v0 - T a0) / (A T),
- * (x1 - x0 - T v0 - T^2 a0 / 2) / (A T^2 / 2)); arrays at original index. */
+/* Training target [N x 6] (sorted): ((v1 - v0) / (A T),
+ * (x1 - x0 - T v0) / (A T^2 / 2)); arrays at original index (a0 unused). */
 int sphdem_mpnn_targets(const sphdem_grid_t *gs, const float *x0, const float *v0,
                         const float *a0, const float *x1, const float *v1, double T,
                         float *target, void *stream);
+/* The outputs of one explicit step of length T from a0 -- the method without
+ * the network -- in the network's output layout: (a0 / A, a0 / A). */
+int sphdem_mpnn_explicit_out(const sphdem_grid_t *gs, const float *a0, float *out,
+                             void *stream);
 /* loss += mean of (out[:, 0:6] - target)^2 (FP64, device), dout = its gradient. */
 int sphdem_mpnn_loss(const float *out, const float *target, int n, double *loss, float *dout,
                      void *stream);
-/* One surrogate call: v += T a + A T out[0:3], x += T v + T^2 a / 2 + A T^2 out[3:6] / 2. */
+/* One surrogate call: v += A T out[0:3], x += T v + A T^2 out[3:6] / 2. */
 int sphdem_mpnn_apply(const sphdem_params_t *p, sphdem_sph_t *P, const sphdem_grid_t *gs,
                       const float *out, double T, void *stream);
 /* Start of a surrogate call: the fused SPH passes of step `step` (Shepard
@@ -358,8 +362,9 @@ int sphdem_surrogate_prep(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_
  * start, and the integration of Kernel_step_dem.  The contact histories go
  * back to the old and new CSR as stage -1 of sphdem_coupled_step leaves them.
  * `scratch` holds sphdem_dem_substeps_scratch(n, precision) bytes.  flags
- * bit 0: a CSR or history row overflowed; bit 1: a DEM moved farther than
- * `skin` from where gd was built (the candidate sets may then be short). */
+ * bit 0: the CSR capacity was exceeded; bit 1: a DEM moved farther than
+ * `skin` from where gd was built (the candidate sets may then be short);
+ * bit 2: a DEM had more than 64 contacts of one kind. */
 long sphdem_dem_substeps_scratch(int n, int precision);
 int sphdem_dem_substeps(const sphdem_params_t *p, sphdem_dem_t *D, sphdem_dembnd_t *W,
                         const sphdem_grid_t *gd, const sphdem_grid_t *gw, int n_sub, double skin,

@@ -107,6 +107,7 @@ _SIGNATURES = {
                               c_void_p, c_void_p], c_int),
     "sphdem_mpnn_targets": ([POINTER(GridStruct), c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_double, c_void_p, c_void_p], c_int),
+    "sphdem_mpnn_explicit_out": ([POINTER(GridStruct), c_void_p, c_void_p, c_void_p], c_int),
     "sphdem_mpnn_loss": ([c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p], c_int),
     "sphdem_mpnn_apply": ([POINTER(Params), POINTER(SphView), POINTER(GridStruct), c_void_p,
                            c_double, c_void_p], c_int),

@@ -634,9 +634,10 @@ static int surrogate_prep(const sphdem_params_t *p, const sphdem_sph_t *P, const
 // live in fixed-width rows (kEll per DEM and kind) in `scratch` and go back
 // to the CSR (old and new, as stage -1 leaves them) at the end.  Loads are
 // the same as the plain DEM stage's (dem_contact, gravity, last-match
-// history); flags bit 0: CSR or row overflow, bit 1: a DEM moved farther
-// than `skin` from where the grid was built.
-constexpr int kEll = 32;
+// history); flags bit 0: CSR capacity exceeded, bit 1: a DEM moved farther
+// than `skin` from where the grid was built, bit 2: a history row exceeded
+// kEll contacts.
+constexpr int kEll = 64;
 constexpr int kSubThreads = 1024;
 
 template <typename T>
@@ -684,7 +685,7 @@ __global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> 
             int c = 0;
             for (int k = o0; k < o1; ++k) {
                 if (c == kEll) {
-                    atomicOr(flags, 1);
+                    atomicOr(flags, 4);
                     break;
                 }
                 const int r = row(0, kind, a) * kEll + c;
@@ -749,7 +750,7 @@ __global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> 
                             ids[rn * kEll + out] = b;
                             st3(tang, rn * kEll + out, disp);
                         } else {
-                            atomicOr(flags, 1);
+                            atomicOr(flags, 4);
                         }
                         ++out;
                     }

@@ -571,18 +571,34 @@ __global__ void k_features(GridV<float> gs, const float *vel, const float *acc, 
     fvel[k] = make_float4(v[0], v[1], v[2], 0.f);
 }
 
-// Training targets in sorted order: the ten-substep change beyond one
-// explicit step of length T from (x0, v0, a0).
+// Training targets in sorted order: the ten-substep velocity change and
+// the displacement beyond the drift T v0, in units of A T and A T^2 / 2.
+// (One explicit step of length T from the plain acceleration a0 -- the
+// method without the network -- is out = (a0 / A, a0 / A).)
 __global__ void k_targets(GridV<float> gs, const float *x0, const float *v0, const float *a0,
                           const float *x1, const float *v1, float T, float *target) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= gs.n) return;
     const int id = gs.ids[k];
     for (int c = 0; c < 3; ++c) {
-        const float a = a0[3 * id + c], v = v0[3 * id + c];
-        target[(size_t)k * 6 + c] = (v1[3 * id + c] - v - T * a) / (kAScale * T);
+        const float v = v0[3 * id + c];
+        target[(size_t)k * 6 + c] = (v1[3 * id + c] - v) / (kAScale * T);
         target[(size_t)k * 6 + 3 + c] =
-            (x1[3 * id + c] - x0[3 * id + c] - T * v - 0.5f * T * T * a) / (0.5f * kAScale * T * T);
+            (x1[3 * id + c] - x0[3 * id + c] - T * v) / (0.5f * kAScale * T * T);
+    }
+    (void)a0;
+}
+
+// The explicit step's outputs (a0 / A, a0 / A), for the ablation.
+__global__ void k_explicit_out(GridV<float> gs, const float *a0, float *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= gs.n) return;
+    const int id = gs.ids[k];
+    for (int c = 0; c < kF; ++c) out[(size_t)k * kF + c] = 0.f;
+    for (int c = 0; c < 3; ++c) {
+        const float a = a0[3 * id + c] * (1.0f / kAScale);
+        out[(size_t)k * kF + c] = a;
+        out[(size_t)k * kF + 3 + c] = a;
     }
 }
 
@@ -604,16 +620,17 @@ __global__ __launch_bounds__(256) void k_loss(const float *out, const float *tar
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, acc / (6.0 * (double)n));
 }
 
-// One surrogate call: dv = T a + A T out[0:3], dx = T v + T^2/2 a + A T^2/2 out[3:6].
+// One surrogate call: dv = A T out[0:3], dx = T v + A T^2/2 out[3:6].
 __global__ void k_apply(GridV<float> gs, const float *out, float T, float *pos, float *vel,
                         const float *acc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= gs.n) return;
     const int id = gs.ids[k];
+    (void)acc;
     for (int c = 0; c < 3; ++c) {
-        const float a = acc[3 * id + c], v = vel[3 * id + c];
-        const float dv = T * a + kAScale * T * out[(size_t)k * kF + c];
-        const float dx = T * v + 0.5f * T * T * a + 0.5f * kAScale * T * T * out[(size_t)k * kF + 3 + c];
+        const float v = vel[3 * id + c];
+        const float dv = kAScale * T * out[(size_t)k * kF + c];
+        const float dx = T * v + 0.5f * kAScale * T * T * out[(size_t)k * kF + 3 + c];
         vel[3 * id + c] = v + dv;
         pos[3 * id + c] = pos[3 * id + c] + dx;
     }
@@ -794,6 +811,17 @@ int sphdem_mpnn_targets(const sphdem_grid_t *gs, const float *x0, const float *v
     hipLaunchKernelGGL(k_targets, dim3(blocks_for(gs->n, 256)), dim3(256), 0, as_stream(stream),
                        view<float>(gs), x0, v0, a0, x1, v1, (float)T, target);
     return check_launch("mpnn_targets");
+}
+
+int sphdem_mpnn_explicit_out(const sphdem_grid_t *gs, const float *a0, float *out,
+                             void *stream) {
+    int rc = mpnn_grid_ok(gs);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(gs->n == 0 || (a0 && out), "NULL buffer");
+    if (gs->n == 0) return SPHDEM_OK;
+    hipLaunchKernelGGL(k_explicit_out, dim3(blocks_for(gs->n, 256)), dim3(256), 0,
+                       as_stream(stream), view<float>(gs), a0, out);
+    return check_launch("mpnn_explicit_out");
 }
 
 int sphdem_mpnn_loss(const float *out, const float *target, int n, double *loss, float *dout,

@@ -9,11 +9,12 @@ relative position, distance, relative velocity, a wall flag and the kernel
 weight.  Two message-passing rounds of width 64 with 3-layer SiLU MLPs, then
 a decoder to six numbers per particle:
 
-    dv = T a + A T out[0:3]
-    dx = T v + T^2/2 a + A T^2/2 out[3:6]        (T = 10 dt, A = 9.81 m/s^2)
+    dv = A T out[0:3]
+    dx = T v + A T^2/2 out[3:6]                  (T = 10 dt, A = 9.81 m/s^2)
 
-so out = 0 is one explicit step of length T from the current acceleration
-(the method without the network).  Training pairs (state, state after ten
+The method without the network is one explicit step of length T from the
+current plain acceleration a, out = (a/A, a/A); the network reads a among
+its inputs and is trained to do better.  Training pairs (state, state after ten
 plain substeps) come from the batched ensemble; the loss is the mean square
 of out - target over particles and the six outputs; Adam on FP32 master
 weights, re-packed to bf16 fragments after every update.
@@ -286,6 +287,8 @@ class Pair:
     rho0: torch.Tensor
     x1: torch.Tensor
     v1: torch.Tensor
+    m: torch.Tensor | None = None      # particle masses
+    flt: torch.Tensor | None = None    # Shepard field held at x0
 
 
 def collect_pairs(ensemble, snapshots: int = 8, interval: int = 40, warm: int = 20,
@@ -299,12 +302,13 @@ def collect_pairs(ensemble, snapshots: int = 8, interval: int = 40, warm: int = 
     for s in range(snapshots):
         x0, v0 = ens.P.pos.clone(), ens.P.vel.clone()
         ens.step()                                   # computes a(x0), rho(x0)
-        a0, rho0 = ens.P.acc.clone(), ens.P.rho.clone()
+        a0, rho0, flt = ens.P.acc.clone(), ens.P.rho.clone(), ens.P.flt.clone()
         ens.run(SUBSTEPS - 1)
         x1, v1 = ens.P.pos.clone(), ens.P.vel.clone()
         for k in range(ens.n):
             sl = ens._slice("fluid", k)
-            pairs.append(Pair(x0[sl], v0[sl], a0[sl], rho0[sl], x1[sl], v1[sl]))
+            pairs.append(Pair(x0[sl], v0[sl], a0[sl], rho0[sl], x1[sl], v1[sl],
+                              ens.P.m[sl], flt[sl]))
         if log:
             log(f"[surrogate] snapshot {s + 1}/{snapshots} at step {ens.step_index}")
         ens.run(max(interval - SUBSTEPS, 0))
@@ -313,10 +317,16 @@ def collect_pairs(ensemble, snapshots: int = 8, interval: int = 40, warm: int = 
 
 
 class Trainer:
-    """Adam on pairs of one scene layout (all pairs share the wall set)."""
+    """Adam on pairs of one scene layout (all pairs share the wall set).
+
+    Training noise (GNS-style): `position_noise` (m) perturbs x0; the plain
+    density and acceleration are then recomputed at the perturbed positions
+    (fused SPH passes, fluid and wall), so the network sees the state a
+    drifting rollout produces, while the target still points at the clean
+    state ten substeps later.  `velocity_noise` (m/s) perturbs v0 likewise."""
 
     def __init__(self, net: MPNN, solv, walls, device="cuda:0", velocity_noise: float = 0.0,
-                 seed: int = 0):
+                 seed: int = 0, position_noise: float = 0.0):
         from .simulation import solver_params
         from .struct import SPHptl
         self.net, self.solv = net, solv
@@ -333,20 +343,55 @@ class Trainer:
         # training noise on the input velocity (m/s): the target keeps the
         # clean future, so the network learns to pull a perturbed state back
         self.noise = float(velocity_noise)
+        self.pos_noise = float(position_noise)
         self._gen = torch.Generator(device=self.dev)
         self._gen.manual_seed(int(seed))
+        if self.pos_noise > 0:
+            from .kernels import Workspace
+            self.B = walls.to(self.dev)
+            self.ws = Workspace(self.dev)
+            self.ws.reserve(1, self.B.n, 1)
+            self._Pn = SPHptl()
+            self._an = self._rn = None
+
+    def _perturbed(self, pair: Pair, v0: torch.Tensor):
+        """x0 + noise, and the plain density / acceleration recomputed there."""
+        n = pair.x0.shape[0]
+        x0 = pair.x0 + self.pos_noise * torch.randn(pair.x0.shape, device=self.dev,
+                                                    generator=self._gen)
+        if self._an is None or self._an.shape[0] < n:
+            self._an, self._rn = _f32((n, 3), self.dev), _f32(n, self.dev)
+            self._pres, self._pgf = _f32(n, self.dev), _f32((n, 3), self.dev)
+            self._por, self._accd = torch.ones(n, device=self.dev), _f32((n, 3), self.dev)
+        self.ws.reserve(n, self.B.n, 1)
+        Pn = self._Pn
+        Pn.pos, Pn.vel, Pn.m, Pn.flt = x0, v0, pair.m, pair.flt
+        Pn.rho, Pn.acc = self._rn[:n], self._an[:n]
+        Pn.rho.copy_(pair.rho0)
+        Pn.pres, Pn.pgf, Pn.porosity, Pn.acc_dem = (self._pres[:n], self._pgf[:n], self._por[:n],
+                                                     self._accd[:n])
+        self.gs.build(x0, self.solv.support, weights=pair.m)
+        self.gb.ensure_weights(self.B.m)
+        _lib.call("sphdem_surrogate_prep", self.params, _lib.sph_view(Pn), _lib.bnd_view(self.B),
+                  None, self.gs.struct(), self.gb.struct(), None, 1, self.ws.struct(),
+                  _lib.stream_handle())
+        return x0, Pn.acc, Pn.rho
 
     def _prepare(self, pair: Pair, noise: bool = False) -> int:
         n = pair.x0.shape[0]
-        self.gs.build(pair.x0, self.solv.support)
         v0 = pair.v0
         if noise and self.noise > 0:
             v0 = v0 + self.noise * torch.randn(v0.shape, device=self.dev, generator=self._gen)
-        self._P.pos, self._P.vel, self._P.acc, self._P.rho = pair.x0, v0, pair.a0, pair.rho0
+        if noise and self.pos_noise > 0 and pair.m is not None:
+            x0, a0, rho0 = self._perturbed(pair, v0)
+        else:
+            x0, a0, rho0 = pair.x0, pair.a0, pair.rho0
+            self.gs.build(x0, self.solv.support)
+        self._P.pos, self._P.vel, self._P.acc, self._P.rho = x0, v0, a0, rho0
         if self.target is None or self.target.shape[0] < n:
             self.target = _f32((n, 6), self.dev)
-        _lib.call("sphdem_mpnn_targets", self.gs.struct(), _lib.fptr(pair.x0),
-                  _lib.fptr(v0), _lib.fptr(pair.a0), _lib.fptr(pair.x1), _lib.fptr(pair.v1),
+        _lib.call("sphdem_mpnn_targets", self.gs.struct(), _lib.fptr(x0),
+                  _lib.fptr(v0), _lib.fptr(a0), _lib.fptr(pair.x1), _lib.fptr(pair.v1),
                   self.T, _lib.fptr(self.target), _lib.stream_handle())
         return n
 
@@ -361,17 +406,23 @@ class Trainer:
         self.net.adam(lr)
 
     def evaluate(self, pairs: list) -> dict:
-        """Mean loss of the network and of out = 0 (one explicit step of
-        length T, the method without the network) over `pairs`."""
+        """Mean loss over `pairs` of the network and of the method without it
+        (one explicit step of length T from the plain acceleration)."""
         tot, base = 0.0, 0.0
+        st = _lib.stream_handle()
         for pr in pairs:
             n = self._prepare(pr)
             out = self.net.forward(self.params, self._P, self.gs, self.gb)
             self.loss.zero_()
             _lib.call("sphdem_mpnn_loss", _lib.fptr(out), _lib.fptr(self.target[:n]), n,
-                      _lib.fptr(self.loss), _lib.fptr(self.net.dout[:n]), _lib.stream_handle())
+                      _lib.fptr(self.loss), _lib.fptr(self.net.dout[:n]), st)
             tot += float(self.loss.item())
-            base += float((self.target[:n].double() ** 2).mean().item())
+            _lib.call("sphdem_mpnn_explicit_out", self.gs.struct(), _lib.fptr(self._P.acc),
+                      _lib.fptr(self.net.dA[:n]), st)
+            self.loss.zero_()
+            _lib.call("sphdem_mpnn_loss", _lib.fptr(self.net.dA[:n]), _lib.fptr(self.target[:n]),
+                      n, _lib.fptr(self.loss), _lib.fptr(self.net.dB[:n]), st)
+            base += float(self.loss.item())
         return {"loss": tot / len(pairs), "loss_without_network": base / len(pairs)}
 
     def train(self, pairs: list, iters: int, lr: float = 1e-3, lr_end: float = 1e-4,
@@ -415,6 +466,8 @@ class RolloutStats:
     rejected: int = 0
     density_err: list = field(default_factory=list)
     velocity_err: list = field(default_factory=list)
+    unhealthy: int = 0
+    flag_bits: int = 0
 
 
 class SurrogateRollout:
@@ -428,6 +481,11 @@ class SurrogateRollout:
         self.eng = CoupledEngine(solv, P, B, D, W, fast_math=fast_math)
         self.net, self.solv = net, solv
         self.judge_every, self.density_tol = int(judge_every), float(density_tol)
+        self.health_tol = 0.02
+        wall = self.eng.B.pos
+        self._box_lo = wall.min(0).values - float(solv.support)
+        self._box_hi = wall.max(0).values + float(solv.support)
+        self._box_hi[2] += 1.0
         self.fallback_calls = int(fallback_calls)
         self.T = SUBSTEPS * float(solv.dt)
         self.stats = RolloutStats()
@@ -530,13 +588,40 @@ class SurrogateRollout:
         self._density(e.P.pos, rho_p)
         rho0 = float(self.solv.rho0)
         d_err = float(torch.sqrt((((rho_s - rho_p) / rho0).double() ** 2).mean()))
-        if int(e.ws.flags[0].item()) & 2:       # a DEM outran the substeps' skin
-            e.ws.flags[0] = int(e.ws.flags[0].item()) & ~2
+        if self._dem_flags() or not self.healthy():
             d_err = float("inf")
         dv = (v_s - e.P.vel).double()
         v_err = float(torch.sqrt((dv ** 2).sum(1).mean())
                       / torch.sqrt((e.P.vel.double() ** 2).sum(1).mean()).clamp_min(1e-30))
         return {"density_err": d_err, "velocity_err": v_err}
+
+    def _dem_flags(self) -> bool:
+        """True (and cleared) when a DEM substep overflowed a contact buffer
+        or a DEM outran the skin since the last look: those calls are invalid."""
+        f = self.eng.ws.flags
+        v = int(f[0].item())
+        if v & 7:
+            f[0] = 0
+            self.stats.flag_bits |= v
+        return bool(v & 7)
+
+    def healthy(self) -> bool:
+        """The current state is finite, every fluid particle is inside the
+        tank's box (1 m of head room above it), no speed exceeds half the
+        sound speed, and the RMS density deviation (Shepard density at the
+        current positions) is below health_tol."""
+        e = self.eng
+        rho = e.P.rho.clone()
+        self._density(e.P.pos, rho)
+        dev = float(torch.sqrt(((rho.double() / float(self.solv.rho0) - 1.0) ** 2).mean()))
+        pos = e.P.pos
+        inside = bool(((pos >= self._box_lo) & (pos <= self._box_hi)).all())
+        vmax = float(e.P.vel.abs().max())
+        ok = (math.isfinite(dev) and dev < self.health_tol and inside and math.isfinite(vmax)
+              and vmax < 0.5 * float(self.solv.c0))
+        if not ok:
+            self.stats.unhealthy += 1
+        return ok
 
     # ---- loop
     def _checkpoint(self) -> None:
@@ -585,12 +670,26 @@ class SurrogateRollout:
                 self._sur_since += 1
         return st
 
+    def finish(self) -> None:
+        """Validate the calls since the last judged one: if a DEM substep
+        flagged them or the state is not healthy, revert and redo them with
+        the plain step."""
+        if self._sur_since and (self._dem_flags() or not self.healthy()):
+            st = self.stats
+            st.rejected += 1
+            redo = self._restore()
+            for _ in range(redo):
+                self.plain_call()
+            st.plain_calls += redo
+            st.surrogate_calls -= self._sur_since
+            self._checkpoint()
+
 
 # ---------------------------------------------------------------- command line
 def train_on_ensemble(n_inst: int = 16, seed: int = 20263, snapshots: int = 8,
                       interval: int = 100, warm: int = 50, iters: int = 2000,
                       lr: float = 1e-3, device="cuda:0", log=None, init_seed: int = 1,
-                      velocity_noise: float = 0.01):
+                      velocity_noise: float = 0.0, position_noise: float = 0.0):
     """Collect pairs from a C3-style ensemble (dx = 0.01) and train a new
     network; returns (net, report).  The last snapshot's pairs are held out."""
     from .ensemble import EnsembleEngine
@@ -604,13 +703,14 @@ def train_on_ensemble(n_inst: int = 16, seed: int = 20263, snapshots: int = 8,
     del ens
     train, held = pairs[:-n_inst], pairs[-n_inst:]
     net = MPNN(device, seed=init_seed)
-    tr = Trainer(net, wl.solv, wl.scenes[0][1], device=device, velocity_noise=velocity_noise)
+    tr = Trainer(net, wl.solv, wl.scenes[0][1], device=device, velocity_noise=velocity_noise,
+                 position_noise=position_noise)
     t0 = time.time()
     hist = tr.train(train, iters, lr=lr, log=log)
     torch.cuda.synchronize()
     t_train = time.time() - t0
     rep = {"instances": n_inst, "pairs": len(pairs), "iters": iters, "data_s": t_data,
-           "velocity_noise": velocity_noise,
+           "velocity_noise": velocity_noise, "position_noise": position_noise,
            "train_s": t_train, "loss_history": hist, "held_out": tr.evaluate(held),
            "train_eval": tr.evaluate(train[:n_inst])}
     return net, rep
@@ -641,6 +741,7 @@ def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st = ro.run(calls - 1)
+    ro.finish()
     torch.cuda.synchronize()
     t_sur = time.perf_counter() - t0
     ro.eng.check()
@@ -652,7 +753,8 @@ def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
            "surrogate_s": t_sur, "plain_sim_per_wall": sim_plain,
            "surrogate_sim_per_wall": sim_sur, "speedup": sim_sur / sim_plain,
            "surrogate_calls": st.surrogate_calls, "plain_calls": st.plain_calls,
-           "judged": st.judged, "rejected": st.rejected,
+           "judged": st.judged, "rejected": st.rejected, "unhealthy": st.unhealthy,
+           "dem_flag_bits": st.flag_bits,
            "density_err_max": max(st.density_err) if st.density_err else None,
            "density_err_mean": float(np.mean(st.density_err)) if st.density_err else None,
            "velocity_err_max": max(st.velocity_err) if st.velocity_err else None,
@@ -673,7 +775,8 @@ def main(argv=None) -> int:
     ap.add_argument("--instances", type=int, default=16)
     ap.add_argument("--iters", type=int, default=2000)
     ap.add_argument("--snapshots", type=int, default=8)
-    ap.add_argument("--noise", type=float, default=0.01, help="training velocity noise, m/s")
+    ap.add_argument("--noise", type=float, default=0.0, help="training velocity noise, m/s")
+    ap.add_argument("--pos-noise", type=float, default=0.0, help="training position noise, m")
     ap.add_argument("--interval", type=int, default=100)
     ap.add_argument("--out", default=None, help="where to save trained weights (.npz)")
     ap.add_argument("--weights", default=None, help="weights to load instead of training")
@@ -690,7 +793,8 @@ def main(argv=None) -> int:
     elif args.train:
         net, rep["train"] = train_on_ensemble(args.instances, iters=args.iters,
                                               snapshots=args.snapshots, interval=args.interval,
-                                              velocity_noise=args.noise, log=log)
+                                              velocity_noise=args.noise,
+                                              position_noise=args.pos_noise, log=log)
         if args.out:
             save_params(net, args.out, {"instances": args.instances, "iters": args.iters})
     else:
