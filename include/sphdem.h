@@ -373,6 +373,14 @@ int sphdem_dem_substeps(const sphdem_params_t *p, sphdem_dem_t *D, sphdem_dembnd
 int sphdem_adam(float *param, const float *grad, float *m, float *v, int n, double lr,
                 double beta1, double beta2, double eps, int step, void *stream);
 
+/* ---- ASCII output (host only) ------------------------------------------ */
+/* The VTK XML frame writer's number formatting (output/output.py:14-24):
+ * "%.6g" floats / "%d" integers, per_line values on a line, single spaces,
+ * '\n' between lines, no trailing newline.  Return the bytes written (without
+ * the NUL) or -1 when `cap` is too small. */
+long sphdem_format_g6(const double *v, long n, int per_line, char *out, long cap);
+long sphdem_format_int(const int *v, long n, int per_line, char *out, long cap);
+
 #ifdef __cplusplus
 }
 #endif

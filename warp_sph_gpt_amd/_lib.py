@@ -118,6 +118,8 @@ _SIGNATURES = {
     "sphdem_dem_substeps": ([POINTER(Params), POINTER(DemView), POINTER(DemBndView),
                              POINTER(GridStruct), POINTER(GridStruct), c_int, c_double, c_void_p,
                              POINTER(WorkStruct), c_void_p], c_int),
+    "sphdem_format_g6": ([c_void_p, ctypes.c_long, c_int, c_void_p, ctypes.c_long], ctypes.c_long),
+    "sphdem_format_int": ([c_void_p, ctypes.c_long, c_int, c_void_p, ctypes.c_long], ctypes.c_long),
     "sphdem_adam": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
                      c_double, c_int, c_void_p], c_int),
     "sphdem_tuning_set": ([POINTER(TuningStruct)], c_int),
