@@ -51,7 +51,7 @@ def test_generator_reproduces_the_reference_input_files(tmp_path):
     paths = generate_files(tmp_path)
     for kind, path in paths.items():
         data = path.read_bytes()
-        assert hashlib.sha256(data).hexdigest() == golden[kind]["sha256"], kind
+        assert hashlib.sha256(data).hexdigest() == golden[kind]["sha256"]["sha256"], kind
         assert data.count(b"\n") - 1 == golden[kind]["rows"]
 
 
@@ -196,3 +196,25 @@ def test_rsa_packing_is_seeded_and_non_overlapping():
     d = np.sqrt(((c[:, None] - c[None]) ** 2).sum(-1))
     np.fill_diagonal(d, 1.0)
     assert len(c) == 400 and np.all(d >= (rc[:, None] + rc[None]) - 1e-12)
+
+
+def test_generator_command_line_writes_the_shipped_case_and_c1(tmp_path):
+    """python -m warp_sph_gpt_amd.input_gen: c0 reproduces the shipped files
+    byte for byte; c1 writes 100,000 fluid and 200 DEM rows that read back."""
+    import hashlib
+    import json
+    from pathlib import Path
+
+    from warp_sph_gpt_amd.config import Solv
+    from warp_sph_gpt_amd.input_gen import main
+    from warp_sph_gpt_amd.input_reader import load_sph_particles
+    golden = json.loads((Path(__file__).parent / "golden" / "reference_input_sha256.json")
+                        .read_text())
+    main(["--output-dir", str(tmp_path / "c0")])
+    for kind in ("SPH", "BND", "DEM", "DEMBND"):
+        data = (tmp_path / "c0" / f"input_{kind}.txt").read_bytes()
+        assert hashlib.sha256(data).hexdigest() == golden[kind]["sha256"]
+    main(["--output-dir", str(tmp_path / "c1"), "--workload", "c1"])
+    s = Solv(device="cpu", input_dir=str(tmp_path / "c1"), h=0.013, dt=5e-5)
+    P, B, D, W = load_sph_particles(s)
+    assert (P.n, B.n, D.n, W.n) == (100_000, 180_456, 200, 58_017)

@@ -425,12 +425,28 @@ def generate_files(output_dir: str | Path,
             for kind, p in particles.items()}
 
 
-def main() -> None:
+def write_structs(output_dir: str | Path, structs) -> dict:
+    """input_{SPH,BND,DEM,DEMBND}.txt of given (P, B, D, W) structs."""
+    return {kind: write_particle_file(Path(output_dir) / f"input_{kind}.txt", kind, p)
+            for kind, p in zip(("SPH", "BND", "DEM", "DEMBND"), structs)}
+
+
+def main(argv=None) -> None:
+    """python -m warp_sph_gpt_amd.input_gen [--output-dir DIR] [--workload c0|c1|c2]
+
+    c0 (default) writes the reference's shipped case; c1 and c2 write the
+    larger dam breaks with their seeded random sequential DEM placement."""
     import argparse
     parser = argparse.ArgumentParser(description="Generate the SPH/DEM dam-break input files")
     parser.add_argument("--output-dir", type=Path, default=Path("input_file"))
-    args = parser.parse_args()
-    for kind, path in generate_files(args.output_dir).items():
+    parser.add_argument("--workload", choices=("c0", "c1", "c2"), default="c0")
+    args = parser.parse_args(argv)
+    if args.workload == "c0":
+        out = generate_files(args.output_dir)
+    else:
+        from .workloads import WORKLOADS
+        out = write_structs(args.output_dir, WORKLOADS[args.workload](device="cpu").structs)
+    for kind, path in out.items():
         print(f"[{kind}] {path}")
 
 
