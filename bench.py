@@ -61,7 +61,9 @@ def main(argv=None) -> int:
                     help="replay captured hipGraphs (always on for the c3 ensemble)")
     ap.add_argument("--instances", type=int, default=64, help="c3: number of instances")
     ap.add_argument("--tune-steps", type=int, default=2,
-                    help="c3: steps per autotuner candidate (0: library default shapes)")
+                    help="steps per autotuner candidate during warm-up (0: library defaults)")
+    ap.add_argument("--surrogate-sim-time", type=float, default=0.1,
+                    help="simulated seconds of the C1 surrogate-vs-plain rollout (0: skip)")
     ap.add_argument("--math", choices=("exact", "fast"), default="fast",
                     help="W/dW arithmetic of the neighbour sums (parity-tested both ways)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for cpu_baseline")
@@ -107,6 +109,10 @@ def main(argv=None) -> int:
     log(f"[bench] {wl.name}: {P.n} fluid, {B.n} wall, {D.n} DEM, {W.n} DEM-wall; {wl.note}; "
         f"setup {time.time() - t0:.1f}s")
     eng = CoupledEngine(wl.solv, P, B, D, W, fast_math=args.math == "fast")
+    tuned = None
+    if args.tune_steps > 0:
+        from warp_sph_gpt_amd import autotune
+        tuned = autotune.autotune(eng.step, steps=args.tune_steps, log=log)["tuning"]
     for _ in range(args.warmup):
         eng.step()
     torch.cuda.synchronize()
@@ -147,6 +153,11 @@ def main(argv=None) -> int:
     if not args.no_cpu_baseline:
         stride = args.cpu_stride or (8 if wl.name == "c2" else 1)
         cpu = cpu_baseline(wl, args.cpu_steps, stride)
+    sur = None
+    if args.surrogate_sim_time > 0:
+        del eng
+        torch.cuda.empty_cache()
+        sur = surrogate_rollout(args.surrogate_sim_time)
 
     line = {
         "metric": "SPH particle-steps/s at 1.56M ptl; surrogate speedup at <=1% rho error",
@@ -156,8 +167,9 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"{wl.name}: coupled WCSPH-DEM dam break", "fluid": P.n,
                    "wall": B.n, "dem": D.n, "dem_wall": W.n, "dt": wl.solv.dt,
-                   "graph": bool(args.graph), "math": args.math, "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu,
+                   "graph": bool(args.graph), "math": args.math, "tuning": tuned,
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "surrogate": sur,
         "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
         "diagnostics_fp64": diag,
     }
@@ -229,6 +241,31 @@ def bench_ensemble(args) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def surrogate_rollout(sim_time: float) -> dict:
+    """C1 (100,000 fluid, 200 DEM): the plain GPU path and the surrogate
+    rollout (judge and fallback every 50 calls, 1 % density) over the same
+    simulated time; simulated seconds per wall second of each and their ratio."""
+    from warp_sph_gpt_amd.surrogate import MPNN, WEIGHTS, load_params, rollout_c1
+    if not os.path.exists(WEIGHTS):
+        log("[bench] surrogate weights missing; skipped")
+        return None
+    net = MPNN(params=load_params(WEIGHTS))
+    r = rollout_c1(net, sim_time=sim_time, judge_every=50, density_tol=0.01)
+    keep = ("sim_time", "calls", "speedup", "plain_sim_per_wall", "surrogate_sim_per_wall",
+            "surrogate_calls", "plain_calls", "judged", "rejected", "velocity_err_mean",
+            "velocity_err_max", "density_err_mean", "density_err_max")
+    out = {k: (round(r[k], 6) if isinstance(r[k], float) and r[k] == r[k]
+               and abs(r[k]) != float("inf") else (None if isinstance(r[k], float) else r[k]))
+           for k in keep}
+    out["workload"] = "c1: 100,000 fluid, 200 DEM; judge every 50 calls at 1% density"
+    out["final_ke_rel"] = abs(r["final_surrogate"]["fluid_ke"] - r["final_plain"]["fluid_ke"]) \
+        / r["final_plain"]["fluid_ke"]
+    out["final_front_rel"] = abs(r["final_surrogate"]["front"] - r["final_plain"]["front"]) \
+        / r["final_plain"]["front"]
+    log(f"[bench] surrogate: {json.dumps(out)}")
+    return out
 
 
 def cpu_baseline(wl, n_steps: int, stride: int) -> dict:

@@ -31,6 +31,7 @@ rollout reverts to plain steps for `fallback_calls` calls.
 from __future__ import annotations
 
 import math
+import os
 import time
 from dataclasses import dataclass, field
 
@@ -716,6 +717,10 @@ def train_on_ensemble(n_inst: int = 16, seed: int = 20263, snapshots: int = 8,
     return net, rep
 
 
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                       "surrogate_c3.npz")
+
+
 def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
                density_tol: float = 0.01, device="cuda:0", log=None) -> dict:
     """C1 dam break (100,000 fluid, 200 DEM): plain GPU path and surrogate
@@ -773,11 +778,12 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="train the surrogate / roll it out on C1")
     ap.add_argument("--train", action="store_true")
     ap.add_argument("--instances", type=int, default=16)
-    ap.add_argument("--iters", type=int, default=2000)
-    ap.add_argument("--snapshots", type=int, default=8)
+    ap.add_argument("--iters", type=int, default=20000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--snapshots", type=int, default=20)
     ap.add_argument("--noise", type=float, default=0.0, help="training velocity noise, m/s")
     ap.add_argument("--pos-noise", type=float, default=0.0, help="training position noise, m")
-    ap.add_argument("--interval", type=int, default=100)
+    ap.add_argument("--interval", type=int, default=500)
     ap.add_argument("--out", default=None, help="where to save trained weights (.npz)")
     ap.add_argument("--weights", default=None, help="weights to load instead of training")
     ap.add_argument("--rollout", action="store_true")
@@ -791,7 +797,7 @@ def main(argv=None) -> int:
     if args.weights:
         net = MPNN(params=load_params(args.weights))
     elif args.train:
-        net, rep["train"] = train_on_ensemble(args.instances, iters=args.iters,
+        net, rep["train"] = train_on_ensemble(args.instances, iters=args.iters, lr=args.lr,
                                               snapshots=args.snapshots, interval=args.interval,
                                               velocity_noise=args.noise,
                                               position_noise=args.pos_noise, log=log)
