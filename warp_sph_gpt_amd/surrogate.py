@@ -290,13 +290,16 @@ class Pair:
     v1: torch.Tensor
     m: torch.Tensor | None = None      # particle masses
     flt: torch.Tensor | None = None    # Shepard field held at x0
+    x2: torch.Tensor | None = None     # the state twenty plain substeps after x0
+    v2: torch.Tensor | None = None
 
 
 def collect_pairs(ensemble, snapshots: int = 8, interval: int = 40, warm: int = 20,
-                  log=None) -> list:
+                  log=None, chain: bool = False) -> list:
     """Advance a batched ensemble (ensemble.EnsembleEngine) with the plain
     step and cut (state, state + 10 substeps) pairs for every instance at
-    `snapshots` times, `interval` steps apart, after `warm` steps."""
+    `snapshots` times, `interval` steps apart, after `warm` steps; with
+    `chain` also the state 20 substeps on (for pushforward training)."""
     ens = ensemble
     ens.run(warm)
     pairs = []
@@ -306,13 +309,18 @@ def collect_pairs(ensemble, snapshots: int = 8, interval: int = 40, warm: int = 
         a0, rho0, flt = ens.P.acc.clone(), ens.P.rho.clone(), ens.P.flt.clone()
         ens.run(SUBSTEPS - 1)
         x1, v1 = ens.P.pos.clone(), ens.P.vel.clone()
+        x2 = v2 = None
+        if chain:
+            ens.run(SUBSTEPS)
+            x2, v2 = ens.P.pos.clone(), ens.P.vel.clone()
         for k in range(ens.n):
             sl = ens._slice("fluid", k)
             pairs.append(Pair(x0[sl], v0[sl], a0[sl], rho0[sl], x1[sl], v1[sl],
-                              ens.P.m[sl], flt[sl]))
+                              ens.P.m[sl], flt[sl], None if x2 is None else x2[sl],
+                              None if v2 is None else v2[sl]))
         if log:
             log(f"[surrogate] snapshot {s + 1}/{snapshots} at step {ens.step_index}")
-        ens.run(max(interval - SUBSTEPS, 0))
+        ens.run(max(interval - SUBSTEPS * (2 if chain else 1), 0))
     ens.check()
     return pairs
 
@@ -327,7 +335,7 @@ class Trainer:
     state ten substeps later.  `velocity_noise` (m/s) perturbs v0 likewise."""
 
     def __init__(self, net: MPNN, solv, walls, device="cuda:0", velocity_noise: float = 0.0,
-                 seed: int = 0, position_noise: float = 0.0):
+                 seed: int = 0, position_noise: float = 0.0, pushforward: float = 0.0):
         from .simulation import solver_params
         from .struct import SPHptl
         self.net, self.solv = net, solv
@@ -345,9 +353,11 @@ class Trainer:
         # clean future, so the network learns to pull a perturbed state back
         self.noise = float(velocity_noise)
         self.pos_noise = float(position_noise)
+        self.pushforward = float(pushforward)
+        self._rng = np.random.default_rng(int(seed) + 1)
         self._gen = torch.Generator(device=self.dev)
         self._gen.manual_seed(int(seed))
-        if self.pos_noise > 0:
+        if self.pos_noise > 0 or self.pushforward > 0:
             from .kernels import Workspace
             self.B = walls.to(self.dev)
             self.ws = Workspace(self.dev)
@@ -357,9 +367,14 @@ class Trainer:
 
     def _perturbed(self, pair: Pair, v0: torch.Tensor):
         """x0 + noise, and the plain density / acceleration recomputed there."""
-        n = pair.x0.shape[0]
         x0 = pair.x0 + self.pos_noise * torch.randn(pair.x0.shape, device=self.dev,
                                                     generator=self._gen)
+        return self._plain_at(pair, x0, v0)
+
+    def _plain_at(self, pair: Pair, x0: torch.Tensor, v0: torch.Tensor):
+        """The plain density and acceleration of the fluid at (x0, v0) (the
+        pair's masses and Shepard field; fluid and wall, no DEM)."""
+        n = pair.x0.shape[0]
         if self._an is None or self._an.shape[0] < n:
             self._an, self._rn = _f32((n, 3), self.dev), _f32(n, self.dev)
             self._pres, self._pgf = _f32(n, self.dev), _f32((n, 3), self.dev)
@@ -396,9 +411,31 @@ class Trainer:
                   self.T, _lib.fptr(self.target), _lib.stream_handle())
         return n
 
+    def _pushforward(self, pair: Pair) -> int:
+        """Inputs from the network's own prediction of x0 + 10 substeps and
+        targets from the plain state 20 substeps after x0: trains the network
+        on the states its rollouts visit (no gradient through the first call)."""
+        n = self._prepare(pair)
+        out = self.net.forward(self.params, self._P, self.gs, self.gb)
+        from .struct import SPHptl
+        Pt = SPHptl(pos=pair.x0.clone(), vel=pair.v0.clone(), acc=pair.a0)
+        _lib.call("sphdem_mpnn_apply", self.params, _lib.sph_view(Pt), self.gs.struct(),
+                  _lib.fptr(out), self.T, _lib.stream_handle())
+        x1, v1 = Pt.pos, Pt.vel
+        a1, rho1 = self._plain_at(pair, x1, v1)
+        self._P.pos, self._P.vel, self._P.acc, self._P.rho = x1, v1, a1, rho1
+        _lib.call("sphdem_mpnn_targets", self.gs.struct(), _lib.fptr(x1), _lib.fptr(v1),
+                  _lib.fptr(a1), _lib.fptr(pair.x2), _lib.fptr(pair.v2), self.T,
+                  _lib.fptr(self.target), _lib.stream_handle())
+        return n
+
     def step(self, pair: Pair, lr: float) -> None:
         """One Adam step on one pair; the loss is added to self.loss."""
-        n = self._prepare(pair, noise=True)
+        if self.pushforward > 0 and pair.x2 is not None and \
+                self._rng.random() < self.pushforward:
+            n = self._pushforward(pair)
+        else:
+            n = self._prepare(pair, noise=True)
         out = self.net.forward(self.params, self._P, self.gs, self.gb)
         _lib.call("sphdem_mpnn_loss", _lib.fptr(out), _lib.fptr(self.target[:n]), n,
                   _lib.fptr(self.loss), _lib.fptr(self.net.dout[:n]), _lib.stream_handle())
@@ -690,7 +727,8 @@ class SurrogateRollout:
 def train_on_ensemble(n_inst: int = 16, seed: int = 20263, snapshots: int = 8,
                       interval: int = 100, warm: int = 50, iters: int = 2000,
                       lr: float = 1e-3, device="cuda:0", log=None, init_seed: int = 1,
-                      velocity_noise: float = 0.0, position_noise: float = 0.0):
+                      velocity_noise: float = 0.0, position_noise: float = 0.0,
+                      pushforward: float = 0.0):
     """Collect pairs from a C3-style ensemble (dx = 0.01) and train a new
     network; returns (net, report).  The last snapshot's pairs are held out."""
     from .ensemble import EnsembleEngine
@@ -698,20 +736,22 @@ def train_on_ensemble(n_inst: int = 16, seed: int = 20263, snapshots: int = 8,
     wl = c3(device=device, seed=seed, n_inst=n_inst)
     t0 = time.time()
     ens = EnsembleEngine(wl.solv, wl.scenes, device=device, fast_math=True)
-    pairs = collect_pairs(ens, snapshots=snapshots, interval=interval, warm=warm, log=log)
+    pairs = collect_pairs(ens, snapshots=snapshots, interval=interval, warm=warm, log=log,
+                          chain=pushforward > 0)
     torch.cuda.synchronize()
     t_data = time.time() - t0
     del ens
     train, held = pairs[:-n_inst], pairs[-n_inst:]
     net = MPNN(device, seed=init_seed)
     tr = Trainer(net, wl.solv, wl.scenes[0][1], device=device, velocity_noise=velocity_noise,
-                 position_noise=position_noise)
+                 position_noise=position_noise, pushforward=pushforward)
     t0 = time.time()
     hist = tr.train(train, iters, lr=lr, log=log)
     torch.cuda.synchronize()
     t_train = time.time() - t0
     rep = {"instances": n_inst, "pairs": len(pairs), "iters": iters, "data_s": t_data,
            "velocity_noise": velocity_noise, "position_noise": position_noise,
+           "pushforward": pushforward,
            "train_s": t_train, "loss_history": hist, "held_out": tr.evaluate(held),
            "train_eval": tr.evaluate(train[:n_inst])}
     return net, rep
@@ -783,6 +823,8 @@ def main(argv=None) -> int:
     ap.add_argument("--snapshots", type=int, default=20)
     ap.add_argument("--noise", type=float, default=0.0, help="training velocity noise, m/s")
     ap.add_argument("--pos-noise", type=float, default=0.0, help="training position noise, m")
+    ap.add_argument("--pushforward", type=float, default=0.0,
+                    help="fraction of steps trained on the network's own first call")
     ap.add_argument("--interval", type=int, default=500)
     ap.add_argument("--out", default=None, help="where to save trained weights (.npz)")
     ap.add_argument("--weights", default=None, help="weights to load instead of training")
@@ -800,7 +842,8 @@ def main(argv=None) -> int:
         net, rep["train"] = train_on_ensemble(args.instances, iters=args.iters, lr=args.lr,
                                               snapshots=args.snapshots, interval=args.interval,
                                               velocity_noise=args.noise,
-                                              position_noise=args.pos_noise, log=log)
+                                              position_noise=args.pos_noise,
+                                              pushforward=args.pushforward, log=log)
         if args.out:
             save_params(net, args.out, {"instances": args.instances, "iters": args.iters})
     else:
