@@ -273,6 +273,23 @@ __device__ __forceinline__ void query_runs(const GridV<T> &g, T px, T py, T pz, 
     }
 }
 
+// Visit slots a, a+step, ... < b in order, loading the records of U slots
+// before evaluating any of them, so that their loads are in flight together
+// (the neighbour passes are bound by load latency); the evaluation order,
+// hence every sum, is the plain loop's.
+template <int U, typename Load, typename Eval>
+__device__ __forceinline__ void each_slot(int a, int b, int step, Load &&load, Eval &&eval) {
+    int s = a;
+    for (; s + (U - 1) * step < b; s += U * step) {
+        decltype(load(0)) c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = load(s + u * step);
+#pragma unroll
+        for (int u = 0; u < U; ++u) eval(s + u * step, c[u]);
+    }
+    for (; s < b; s += step) eval(s, load(s));
+}
+
 // ---------------------------------------------------------------- wave64 sums
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

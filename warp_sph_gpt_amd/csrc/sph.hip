@@ -40,8 +40,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
     const T sup = p.support, ss = sup * sup, h = p.h;
     T flt = (T)0, rho = (T)0;
     query_runs(gs, me.x, me.y, me.z, sup, [&](int a, int b) {
-        for (int s = a; s < b; ++s) {
-            const V4<T> q = gs.s[s];
+        each_slot<4>(a, b, 1, [&](int s) { return gs.s[s]; }, [&](int s, const V4<T> &q) {
             const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
             const T td = dx * dx + dy * dy + dz * dz;
             if (td < ss) {
@@ -49,11 +48,10 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
                 if (SHEP) flt = flt + (q.w / P.rho[gs.ids[s]]) * wv;
                 if (DENS) rho = rho + q.w * wv;
             }
-        }
+        });
     });
     query_runs(gb, me.x, me.y, me.z, sup, [&](int a, int b) {
-        for (int s = a; s < b; ++s) {
-            const V4<T> q = gb.s[s];
+        each_slot<4>(a, b, 1, [&](int s) { return gb.s[s]; }, [&](int s, const V4<T> &q) {
             const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
             const T td = dx * dx + dy * dy + dz * dz;
             if (td < ss) {
@@ -61,7 +59,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
                 if (SHEP) flt = flt + (q.w / B.rho[gb.ids[s]]) * wv;
                 if (DENS) rho = rho + q.w * wv;
             }
-        }
+        });
     });
     if (SHEP) P.flt[i] = flt;
     if (DENS) {
@@ -95,12 +93,11 @@ __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, 
     T rho = (T)0;
     auto acc = [&](const GridV<T> &g) {
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
-            for (int s = a; s < b; ++s) {
-                const V4<T> q = g.s[s];
+            each_slot<4>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int, const V4<T> &q) {
                 const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
                 const T td = dx * dx + dy * dy + dz * dz;
                 if (td < ss) rho = rho + q.w * wendland_w(msqrt(td), h);
-            }
+            });
         });
     };
     acc(gs);
@@ -145,8 +142,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
     V3<T> acc = mk3<T>(0, 0, 0), pgf = mk3<T>(0, 0, 0);
     auto body = [&](const GridV<T> &g, const T *nrho, const T *npres, const V4<T> *nvel) {
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
-            for (int s = a; s < b; ++s) {
-                const V4<T> q = g.s[s];
+            each_slot<4>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int s, const V4<T> &q) {
                 const V3<T> rij = mk3(me.x - q.x, me.y - q.y, me.z - q.z);
                 const T td = dot(rij, rij);
                 if (td > r2min && td < ss) {
@@ -164,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
                         pgf = pgf - ((q.w / rhoj) * (presj - presi)) * dwp;
                     }
                 }
-            }
+            });
         });
     };
     body(gs, w.fl_rho, w.fl_pres, w.fl_vel);

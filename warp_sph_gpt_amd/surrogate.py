@@ -369,7 +369,7 @@ class Trainer:
         """x0 + noise, and the plain density / acceleration recomputed there."""
         x0 = pair.x0 + self.pos_noise * torch.randn(pair.x0.shape, device=self.dev,
                                                     generator=self._gen)
-        return self._plain_at(pair, x0, v0)
+        return (x0, *self._plain_at(pair, x0, v0))
 
     def _plain_at(self, pair: Pair, x0: torch.Tensor, v0: torch.Tensor):
         """The plain density and acceleration of the fluid at (x0, v0) (the
@@ -391,7 +391,7 @@ class Trainer:
         _lib.call("sphdem_surrogate_prep", self.params, _lib.sph_view(Pn), _lib.bnd_view(self.B),
                   None, self.gs.struct(), self.gb.struct(), None, 1, self.ws.struct(),
                   _lib.stream_handle())
-        return x0, Pn.acc, Pn.rho
+        return Pn.acc, Pn.rho
 
     def _prepare(self, pair: Pair, noise: bool = False) -> int:
         n = pair.x0.shape[0]
@@ -519,7 +519,7 @@ class SurrogateRollout:
         self.eng = CoupledEngine(solv, P, B, D, W, fast_math=fast_math)
         self.net, self.solv = net, solv
         self.judge_every, self.density_tol = int(judge_every), float(density_tol)
-        self.health_tol = 0.02
+        self.health_tol = 0.01
         wall = self.eng.B.pos
         self._box_lo = wall.min(0).values - float(solv.support)
         self._box_hi = wall.max(0).values + float(solv.support)
@@ -645,18 +645,21 @@ class SurrogateRollout:
 
     def healthy(self) -> bool:
         """The current state is finite, every fluid particle is inside the
-        tank's box (1 m of head room above it), no speed exceeds half the
-        sound speed, and the RMS density deviation (Shepard density at the
-        current positions) is below health_tol."""
+        tank's box (1 m of head room above it), no speed exceeds a third of
+        the sound speed, and the Shepard density at the current positions
+        deviates from rho0 by less than health_tol in RMS and 15 % at most
+        (the plain path keeps about 0.4 % and 8 %)."""
         e = self.eng
         rho = e.P.rho.clone()
         self._density(e.P.pos, rho)
-        dev = float(torch.sqrt(((rho.double() / float(self.solv.rho0) - 1.0) ** 2).mean()))
+        d = rho.double() / float(self.solv.rho0) - 1.0
+        dev = float(torch.sqrt((d ** 2).mean()))
+        dmax = float(d.abs().max())
         pos = e.P.pos
         inside = bool(((pos >= self._box_lo) & (pos <= self._box_hi)).all())
         vmax = float(e.P.vel.abs().max())
-        ok = (math.isfinite(dev) and dev < self.health_tol and inside and math.isfinite(vmax)
-              and vmax < 0.5 * float(self.solv.c0))
+        ok = (math.isfinite(dev) and dev < self.health_tol and dmax < 0.15 and inside
+              and math.isfinite(vmax) and vmax < float(self.solv.c0) / 3.0)
         if not ok:
             self.stats.unhealthy += 1
         return ok
