@@ -51,7 +51,7 @@ def test_generator_reproduces_the_reference_input_files(tmp_path):
     paths = generate_files(tmp_path)
     for kind, path in paths.items():
         data = path.read_bytes()
-        assert hashlib.sha256(data).hexdigest() == golden[kind]["sha256"]["sha256"], kind
+        assert hashlib.sha256(data).hexdigest() == golden[kind]["sha256"], kind
         assert data.count(b"\n") - 1 == golden[kind]["rows"]
 
 
