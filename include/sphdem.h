@@ -373,6 +373,95 @@ int sphdem_dem_substeps(const sphdem_params_t *p, sphdem_dem_t *D, sphdem_dembnd
 int sphdem_adam(float *param, const float *grad, float *m, float *v, int n, double lr,
                 double beta1, double beta2, double eps, int step, void *stream);
 
+/* ---- Eulerian ISPH lid-driven cavity ------------------------------------ */
+/* input/struct_eisph.py:4-28 EISPHptl: fixed fluid or ghost points.  vec3
+ * fields [n*3]; kgc [n*4] the row-major 2x2 correction; mirror int32 [n]
+ * (a ghost's stable fluid row).  residual[0] holds the right-hand-side sum
+ * after the pressure assembly, as in the reference. */
+typedef struct {
+    void *pos, *vel, *vel_star, *vel_bc, *acc, *kgc, *rho, *m, *pres, *Aij, *bi,
+         *Aijpj, *divergence, *residual;
+    int *mirror;
+    int n;
+} sphdem_eisph_t;
+
+/* Solver constants of the cavity step (input/Config.py:11-36: h, rho0, nu,
+ * dt; support = 2h). */
+typedef struct {
+    int precision;   /* 4 = FP32, 8 = FP64 */
+    double h, support, rho0, nu, dt;
+} sphdem_eisph_params_t;
+
+/* Fixed neighbour list of the fluid points (no reference counterpart: the
+ * reference re-walks both HashGrids in every pass, KERNEL_EISPH_*.py; the
+ * points never move, so the walk is done once).  Row i lists, in the
+ * HashGrid visiting order, the fluid and then the ghost neighbours j of
+ * fluid point i with 1e-12 < |r_ij|^2 < support^2: id[e] = j for a fluid
+ * point, -1 - bj for ghost bj, and geo[e] = (gx, gz, A_ij, V_j), the
+ * KGC-corrected grad_i W_ij (KERNEL_EISPH_KNL.py:52-64), the Laplacian
+ * weight 2 V_j (r_ij . g)/r^2 (KERNEL_EISPH_KNL.py:67-71) and V_j = m/rho.
+ * The list holds the kgc, m and rho of its build: rebuild it after changing
+ * any of them or the positions. */
+typedef struct {
+    int n;           /* fluid points of the build */
+    int *offset;     /* [n+1] row starts; offset[n] = entries */
+    int *id;         /* [cap] */
+    void *geo;       /* [cap*4] REAL */
+    int cap;
+    int *count;      /* [n+1] build scratch */
+    int *scan_tmp;   /* [sphdem_scan_scratch(n+1)] */
+    double *part;    /* [sphdem_eisph_part_size(n_fluid, n_ghost)]: the FP64
+                        right-hand-side sum, the raw bi[0], per-workgroup sums */
+    int *done;       /* [1] workgroup counter of that sum; zero before the first
+                        call, every call leaves it zero */
+} sphdem_eisph_nbr_t;
+long sphdem_eisph_part_size(int n_fluid, int n_ghost);
+
+/* KERNEL_EISPH_KNL.py:74-132 Kernel_prepare_kgc: P->kgc from both grids
+ * (built from P->pos and B->pos with radius = support). */
+int sphdem_eisph_kgc(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, const sphdem_eisph_t *B,
+                     const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream);
+/* Neighbour list build, in two calls: _count writes L->offset; the caller
+ * reads offset[n] back, sizes id/geo to cap >= offset[n], then _fill writes
+ * id/geo (it never writes an entry at or beyond cap). */
+int sphdem_eisph_nbr_count(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P,
+                           const sphdem_eisph_t *B, const sphdem_grid_t *gs,
+                           const sphdem_grid_t *gb, sphdem_eisph_nbr_t *L, void *stream);
+int sphdem_eisph_nbr_fill(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P,
+                          const sphdem_eisph_t *B, const sphdem_grid_t *gs,
+                          const sphdem_grid_t *gb, sphdem_eisph_nbr_t *L, void *stream);
+/* KERNEL_EISPH_BC.py:6-31 Kernel_Dirichlet_BC: B->vel (predicted == 0) or
+ * B->vel_star = 2 vel_bc - (fluid field)[mirror]. */
+int sphdem_eisph_dirichlet(int precision, const sphdem_eisph_t *P, sphdem_eisph_t *B,
+                           int predicted, void *stream);
+/* KERNEL_EISPH_BC.py:34-47 Kernel_pressure_boundary_eisph: B->pres = P->pres[mirror]. */
+int sphdem_eisph_pressure_boundary(int precision, const sphdem_eisph_t *P, sphdem_eisph_t *B,
+                                   void *stream);
+/* The pass kernels read the ghost fields as stored in B, as the reference's
+ * kernels do.  KERNEL_EISPH_force.py:12-72 Kernel_advec_vis_eisph. */
+int sphdem_eisph_advec_vis(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                           const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream);
+/* KERNEL_EISPH_PPE.py:12-79 Kernel_build_ppe_eisph, plus the FP64 sum of
+ * the raw bi (EISPH.py:72) into L->part and P->residual[0]. */
+int sphdem_eisph_build_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                           const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream);
+/* KERNEL_EISPH_PPE.py:82-125 Kernel_remove_bi_mean + Kernel_solve_ppe_eisph,
+ * from the sums of the last sphdem_eisph_build_ppe on L. */
+int sphdem_eisph_solve_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                           const sphdem_eisph_nbr_t *L, void *stream);
+/* KERNEL_EISPH_step.py:12-61 Kernel_correct_eisph. */
+int sphdem_eisph_correct(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                         const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream);
+/* KERNEL_EISPH_step.py:64-121 Kernel_diagnostics_eisph. */
+int sphdem_eisph_diagnostics(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                             const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream);
+/* source/EISPH.py:26-90 EISPH_OneStep in five launches: the ghost values
+ * (2 vel_bc - u[mirror], p[mirror]) are formed where they are read, and the
+ * ghost fields of B are published by the pass after their source is final,
+ * so every field ends the step as the reference's ten launches leave it. */
+int sphdem_eisph_step(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem_eisph_t *B,
+                      const sphdem_eisph_nbr_t *L, int diagnostics, void *stream);
+
 /* ---- ASCII output (host only) ------------------------------------------ */
 /* The VTK XML frame writer's number formatting (output/output.py:14-24):
  * "%.6g" floats / "%d" integers, per_line values on a line, single spaces,

@@ -48,6 +48,12 @@ typedef struct {                                                               \
     int n;                                                                     \
 } ora_dembnd##SFX;                                                             \
 typedef struct {                                                               \
+    REAL *pos, *vel, *vel_star, *vel_bc, *acc, *kgc, *rho, *m, *pres, *Aij,    \
+         *bi, *Aijpj, *divergence, *residual;                                  \
+    int *mirror;                                                               \
+    int n;                                                                     \
+} ora_eisph##SFX;                                                              \
+typedef struct {                                                               \
     int dim_x, dim_y, dim_z;                                                   \
     REAL cell_width_inv;                                                       \
     int n;                                                                     \

@@ -638,6 +638,9 @@ void FN(ora_interaction_sph)(FN(ora_sph) *P, const FN(ora_dem) *D, const FN(ora_
     }
 }
 
+/* The Eulerian ISPH cavity (same helpers, same precision). */
+#include "eisph_impl.h"
+
 #undef V3
 #undef WW
 #undef DW

@@ -1,5 +1,5 @@
 """Write tests/golden/reference_input_sha256.json: sha256 and row counts of the
-dam-break input files shipped with the reference (input_file/input_*.txt).
+dam-break and cavity input files shipped with the reference (input_file/input_*.txt).
 Run in a checkout where the reference is available at REFERENCE_ROOT."""
 import hashlib
 import json
@@ -8,7 +8,7 @@ from pathlib import Path
 
 ROOT = Path(os.environ.get("REFERENCE_ROOT", "/root/reference"))
 out = {}
-for kind in ("SPH", "BND", "DEM", "DEMBND"):
+for kind in ("SPH", "BND", "DEM", "DEMBND", "EISPH", "EISPHBND"):
     data = (ROOT / "input_file" / f"input_{kind}.txt").read_bytes()
     out[kind] = {"sha256": hashlib.sha256(data).hexdigest(), "rows": data.count(b"\n") - 1,
                  "bytes": len(data)}

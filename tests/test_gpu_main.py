@@ -63,3 +63,18 @@ def test_engine_and_step_functions_reach_the_same_state(tmp_path):
     for k in (0, 2):
         for f in ("pos", "vel"):
             np.testing.assert_array_equal(np_(getattr(a[k], f)), np_(getattr(b[k], f)))
+
+
+def test_command_line_reads_the_input_files(tmp_path):
+    """main.py from files: the generator's shipped dam-break case, read back
+    and stepped on the GPU, frames at step 0 and the last step."""
+    from warp_sph_gpt_amd.input_gen import CoupledGenerationConfig, generate_files
+    from warp_sph_gpt_amd.main import main
+    gen = CoupledGenerationConfig(device="cpu")
+    generate_files(tmp_path / "in", gen)
+    out = tmp_path / "out"
+    with contextlib.redirect_stdout(io.StringIO()):
+        main(["--device", DEV, "--input-dir", str(tmp_path / "in"), "--steps", "2",
+              "--output-step", "5", "--no-gif", "--output-dir", str(out), "--dt", str(gen.dt)])
+    ds = ET.parse(str(out / "sph.pvd")).findall(".//DataSet")
+    assert [float(d.get("timestep")) for d in ds] == pytest.approx([0.0, 2 * gen.dt])

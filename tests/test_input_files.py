@@ -218,3 +218,11 @@ def test_generator_command_line_writes_the_shipped_case_and_c1(tmp_path):
     s = Solv(device="cpu", input_dir=str(tmp_path / "c1"), h=0.013, dt=5e-5)
     P, B, D, W = load_sph_particles(s)
     assert (P.n, B.n, D.n, W.n) == (100_000, 180_456, 200, 58_017)
+
+
+def test_parsed_fields_are_packed_row_major():
+    """The kernels take packed (N, 3) arrays: every parsed field is contiguous."""
+    for kind in INPUT_FIELDS:
+        got = input_parser(GOLDEN / "reference_input_head" / f"input_{kind}.txt")
+        for name, _ in got.FIELDS:
+            assert getattr(got, name).is_contiguous(), f"{kind}.{name}"

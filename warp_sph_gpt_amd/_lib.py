@@ -77,6 +77,25 @@ class WorkStruct(ctypes.Structure):
 NUM_PASSES = 11
 
 
+EISPH_FIELDS = ("pos", "vel", "vel_star", "vel_bc", "acc", "kgc", "rho", "m", "pres", "Aij",
+                "bi", "Aijpj", "divergence", "residual")
+
+
+class EisphView(ctypes.Structure):
+    _fields_ = _ptr_fields(EISPH_FIELDS) + [("mirror", c_void_p), ("n", c_int)]
+
+
+class EisphParams(ctypes.Structure):
+    _fields_ = [("precision", c_int), ("h", c_double), ("support", c_double),
+                ("rho0", c_double), ("nu", c_double), ("dt", c_double)]
+
+
+class EisphNbr(ctypes.Structure):
+    _fields_ = [("n", c_int), ("offset", c_void_p), ("id", c_void_p), ("geo", c_void_p),
+                ("cap", c_int), ("count", c_void_p), ("scan_tmp", c_void_p), ("part", c_void_p),
+                ("done", c_void_p)]
+
+
 class TuningStruct(ctypes.Structure):
     _fields_ = [("block", c_int * NUM_PASSES), ("xcd_remap", c_int * NUM_PASSES),
                 ("por_x_tile", c_int)]
@@ -118,6 +137,30 @@ _SIGNATURES = {
     "sphdem_dem_substeps": ([POINTER(Params), POINTER(DemView), POINTER(DemBndView),
                              POINTER(GridStruct), POINTER(GridStruct), c_int, c_double, c_void_p,
                              POINTER(WorkStruct), c_void_p], c_int),
+    "sphdem_eisph_part_size": ([c_int, c_int], ctypes.c_long),
+    "sphdem_eisph_kgc": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(GridStruct),
+                          POINTER(GridStruct), c_void_p], c_int),
+    "sphdem_eisph_nbr_count": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView),
+                                POINTER(GridStruct), POINTER(GridStruct), POINTER(EisphNbr),
+                                c_void_p], c_int),
+    "sphdem_eisph_nbr_fill": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView),
+                               POINTER(GridStruct), POINTER(GridStruct), POINTER(EisphNbr),
+                               c_void_p], c_int),
+    "sphdem_eisph_dirichlet": ([c_int, POINTER(EisphView), POINTER(EisphView), c_int, c_void_p],
+                               c_int),
+    "sphdem_eisph_pressure_boundary": ([c_int, POINTER(EisphView), POINTER(EisphView),
+                                        c_void_p], c_int),
+    "sphdem_eisph_advec_vis": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(EisphNbr),
+                                c_void_p], c_int),
+    "sphdem_eisph_build_ppe": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(EisphNbr),
+                                c_void_p], c_int),
+    "sphdem_eisph_solve_ppe": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphNbr), c_void_p], c_int),
+    "sphdem_eisph_correct": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(EisphNbr),
+                              c_void_p], c_int),
+    "sphdem_eisph_diagnostics": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView),
+                                  POINTER(EisphNbr), c_void_p], c_int),
+    "sphdem_eisph_step": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(EisphNbr), c_int,
+                           c_void_p], c_int),
     "sphdem_format_g6": ([c_void_p, ctypes.c_long, c_int, c_void_p, ctypes.c_long], ctypes.c_long),
     "sphdem_format_int": ([c_void_p, ctypes.c_long, c_int, c_void_p, ctypes.c_long], ctypes.c_long),
     "sphdem_adam": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
@@ -285,6 +328,20 @@ def bnd_view(B) -> BndView:
         setattr(v, name, ptr(getattr(B, name), dt, f"P_bnd.{name}"))
     v.n = B.n
     return v
+
+
+def eisph_view(S, what="P_sph") -> EisphView:
+    dt = _real_dtype(S)
+    v = EisphView()
+    for name in EISPH_FIELDS:
+        setattr(v, name, ptr(getattr(S, name), dt, f"{what}.{name}"))
+    v.mirror = ptr(S.mirror, torch.int32, f"{what}.mirror")
+    v.n = S.n
+    return v
+
+
+def eisph_params(precision: int, h: float, rho0: float, nu: float, dt: float) -> EisphParams:
+    return EisphParams(precision, float(h), 2.0 * float(h), float(rho0), float(nu), float(dt))
 
 
 def dem_view(D) -> DemView:

@@ -14,7 +14,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libsphdem.so"
-SOURCES = ("grid.hip", "sph.hip", "dem.hip", "porosity.hip", "mpnn.hip", "format.hip")
+SOURCES = ("grid.hip", "sph.hip", "dem.hip", "porosity.hip", "mpnn.hip", "eisph.hip",
+           "format.hip")
 
 # -ffp-contract=off: every a*b+c rounds twice, as the reference's FP32
 # expressions are written.  Packed FP32 instructions are disabled.

@@ -1,9 +1,9 @@
 """Solver controls of the coupled WCSPH-DEM step.
 
-Mirrors input/Config.py:11-93 (`Solv` fields, `support`, validation) and
-input/Config_SPH_DEM.py:6-15 (the WCSPH-DEM defaults).  Geometry and
-particle properties come from the input files; only solver controls live
-here.
+Mirrors input/Config.py:11-93 (`Solv` fields, `support`, validation; the
+EISPH cavity defaults, here `EISPHSolv`) and input/Config_SPH_DEM.py:6-15
+(the WCSPH-DEM defaults, here `Solv`).  Geometry and particle properties
+come from the input files; only solver controls live here.
 """
 from __future__ import annotations
 
@@ -77,6 +77,13 @@ class BaseSolv:
             raise ValueError("dt exceeds the initial SPH acoustic limit 0.25*h/c0")
         self.validate_dem()
 
+    def validate_eisph(self) -> None:
+        """input/Config.py:89-93: EISPH controls; the loaded points supply the
+        geometry and the wall speeds."""
+        self._validate_common()
+        if not math.isfinite(self.nu) or self.nu < 0.0:
+            raise ValueError("nu must be finite and nonnegative")
+
     def validate_dem(self) -> None:
         """input/Config.py:80-87: coupling controls."""
         if not self.dem_enable:
@@ -97,3 +104,9 @@ class Solv(BaseSolv):
     n_steps: int = 9000
     output_step: int = 90
     dem_enable: bool = True
+
+
+@dataclass
+class EISPHSolv(BaseSolv):
+    """The Eulerian ISPH lid-driven cavity defaults (input/Config.py:11-45):
+    h = 0.052, rho0 = 1, nu = 0.01, dt = 2e-3, 5000 steps, a frame every 100."""

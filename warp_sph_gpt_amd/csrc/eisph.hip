@@ -1,0 +1,618 @@
+// eisph.hip -- the Eulerian ISPH lid-driven cavity (source/EISPH.py,
+// kernel/KERNEL_EISPH_*.py) for gfx950.
+//
+// The points never move, so both HashGrids are walked once: the KGC pass
+// (Kernel_prepare_kgc) and a two-pass build of a fixed neighbour list whose
+// entries carry the neighbour id and the per-pair geometry every later pass
+// needs (corrected gradient, Laplacian weight, neighbour volume), in the
+// grid's visiting order.  A step then streams the list: each pass adds the
+// same terms in the same order as the reference's grid walk, so the sums
+// are the reference's sums.
+//
+// Step (sphdem_eisph_step), five launches, one thread per point, ghosts
+// after the fluid points:
+//   advec    u* = u + dt (-(u.grad)u + nu lap u)
+//   ppe      div u*, sum A_ij, sum A_ij p_j^t, b = rho0 div/dt, the FP64
+//            sum of b (last workgroup to finish adds the workgroup sums in a
+//            fixed order); ghosts publish vel_star
+//   solve    b -= mean(b); p = (b + Aijpj)/Aij - gauge
+//   correct  u = u* - dt grad p / rho; ghosts publish pres
+//   finish   ghosts publish vel; fluid diagnostics on request
+// A ghost's velocity 2 vel_bc - u[mirror] and pressure p[mirror] are formed
+// where they are read: the same values the reference's Dirichlet and
+// Neumann launches store before each read.
+#include "common.hpp"
+
+namespace sphdem {
+namespace {
+
+constexpr int kEisThreads = 256;
+
+template <typename T> struct EisV {
+    T *pos, *vel, *vel_star, *vel_bc, *acc, *kgc, *rho, *m, *pres, *Aij, *bi, *Aijpj, *div, *res;
+    const int *mirror;
+    int n;
+};
+template <typename T> EisV<T> eview(const sphdem_eisph_t *a) {
+    return {(T *)a->pos, (T *)a->vel, (T *)a->vel_star, (T *)a->vel_bc, (T *)a->acc, (T *)a->kgc,
+            (T *)a->rho, (T *)a->m, (T *)a->pres, (T *)a->Aij, (T *)a->bi, (T *)a->Aijpj,
+            (T *)a->divergence, (T *)a->residual, a->mirror, a->n};
+}
+template <typename T> struct NbrV {
+    const int *off, *id;
+    const V4<T> *geo;
+};
+template <typename T> NbrV<T> nview(const sphdem_eisph_nbr_t *L) {
+    return {L->offset, L->id, (const V4<T> *)L->geo};
+}
+
+// ---------------------------------------------------------------- kernels of the pair
+// KERNEL_EISPH_KNL.py:34-49 -- 2-D Wendland C2 dW/dr, left to right.
+template <typename T> __device__ __forceinline__ T wendland2_dw(T r, T h) {
+    T q = r / h;
+    T v = (T)0;
+    if (q < (T)2.0) {
+        T u = (T)1.0 - (T)0.5 * q;
+        v = (T)-35.0 * q * u * u * u / ((T)4.0 * (T)SPHDEM_PI * h * h * h);
+    }
+    return v;
+}
+// KERNEL_EISPH_KNL.py:52-56
+template <typename T> __device__ __forceinline__ V3<T> grad_w(V3<T> rij, T r2, T h) {
+    T r = msqrt(r2);
+    return (wendland2_dw(r, h) * rij) / r;
+}
+// KERNEL_EISPH_KNL.py:59-64
+template <typename T> __device__ __forceinline__ V3<T> corrected(const T c[4], V3<T> g) {
+    return mk3(c[0] * g.x + c[1] * g.z, (T)0.0, c[2] * g.x + c[3] * g.z);
+}
+template <typename T> __device__ __forceinline__ bool in_support(T r2, T support) {
+    return r2 > (T)1.0e-12 && r2 < support * support;  // KERNEL_EISPH_KNL.py:10 R2_EPS
+}
+
+// Walk the fluid then the ghost grid around point i in Warp's order; fn(side,
+// j, rij, r2) for every pair inside the support.
+template <typename T, typename Fn>
+__device__ __forceinline__ void walk_pairs(const GridV<T> &gs, const GridV<T> &gb, V3<T> ri,
+                                           T support, Fn &&fn) {
+    for (int side = 0; side < 2; ++side) {
+        const GridV<T> &g = side == 0 ? gs : gb;
+        query_runs(g, ri.x, ri.y, ri.z, support, [&](int a, int b) {
+            for (int s = a; s < b; ++s) {
+                const V4<T> q = g.s[s];
+                const V3<T> rij = ri - xyz(q);
+                const T r2 = dot(rij, rij);
+                if (in_support(r2, support)) fn(side, g.ids[s], rij, r2);
+            }
+        });
+    }
+}
+
+// KERNEL_EISPH_KNL.py:74-132 Kernel_prepare_kgc
+template <typename T>
+__global__ void k_eis_kgc(EisV<T> P, EisV<T> B, GridV<T> gs, GridV<T> gb, T support, T h) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= P.n) return;
+    const int i = gs.ids[tid];
+    const V3<T> ri = ld3(P.pos, i);
+    T m00 = 0, m01 = 0, m10 = 0, m11 = 0;
+    walk_pairs(gs, gb, ri, support, [&](int side, int j, V3<T> rij, T r2) {
+        const EisV<T> &S = side == 0 ? P : B;
+        const V3<T> g = grad_w(rij, r2, h);
+        const T vol = S.m[j] / S.rho[j];
+        m00 = m00 - vol * rij.x * g.x;
+        m01 = m01 - vol * rij.x * g.z;
+        m10 = m10 - vol * rij.z * g.x;
+        m11 = m11 - vol * rij.z * g.z;
+    });
+    const T det = m00 * m11 - m01 * m10;
+    T c[4] = {(T)1, (T)0, (T)0, (T)1};
+    if (mabs(det) > (T)1.0e-8) {
+        const T inv = (T)1.0 / det;
+        c[0] = m11 * inv; c[1] = -m01 * inv; c[2] = -m10 * inv; c[3] = m00 * inv;
+    }
+    for (int k = 0; k < 4; ++k) P.kgc[4 * i + k] = c[k];
+}
+
+// Entries per fluid point; count[n] = 0 so the exclusive scan of n+1 gives
+// offset[n] = total.
+template <typename T>
+__global__ void k_eis_count(EisV<T> P, GridV<T> gs, GridV<T> gb, T support, int *count) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) count[P.n] = 0;
+    if (tid >= P.n) return;
+    const int i = gs.ids[tid];
+    int c = 0;
+    walk_pairs(gs, gb, ld3(P.pos, i), support, [&](int, int, V3<T>, T) { ++c; });
+    count[i] = c;
+}
+
+template <typename T>
+__global__ void k_eis_fill(EisV<T> P, EisV<T> B, GridV<T> gs, GridV<T> gb, T support, T h,
+                           const int *off, int *id, V4<T> *geo, int cap) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= P.n) return;
+    const int i = gs.ids[tid];
+    T c[4];
+    for (int k = 0; k < 4; ++k) c[k] = P.kgc[4 * i + k];
+    int e = off[i];
+    walk_pairs(gs, gb, ld3(P.pos, i), support, [&](int side, int j, V3<T> rij, T r2) {
+        const EisV<T> &S = side == 0 ? P : B;
+        const V3<T> g = corrected(c, grad_w(rij, r2, h));
+        const T vol = S.m[j] / S.rho[j];
+        // KERNEL_EISPH_KNL.py:67-71 _laplace_weight
+        const T lap = (T)2.0 * vol * dot(rij, g) / r2;
+        if (e < cap) {
+            id[e] = side == 0 ? j : -1 - j;
+            geo[e] = V4<T>{g.x, g.z, lap, vol};
+        }
+        ++e;
+    });
+}
+
+// ---------------------------------------------------------------- the step's passes
+// A ghost's field as the pass reads it: stored in B (kFly false, the
+// reference kernels' reads) or formed from its mirror (kFly true).
+template <typename T, bool kFly>
+__device__ __forceinline__ V3<T> ghost_vec(const EisV<T> &B, const T *fluid, const T *stored,
+                                           int b) {
+    if (kFly) return (T)2.0 * ld3(B.vel_bc, b) - ld3(fluid, B.mirror[b]);
+    return ld3(stored, b);
+}
+template <typename T> __device__ __forceinline__ V3<T> gvec(const V4<T> &G) {
+    return mk3(G.x, (T)0.0, G.y);
+}
+
+// KERNEL_EISPH_force.py:12-72 Kernel_advec_vis_eisph
+template <typename T, bool kFly>
+__global__ void __launch_bounds__(kEisThreads)
+k_eis_advec(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const V3<T> vi = ld3(P.vel, i);
+    V3<T> conv = mk3((T)0, (T)0, (T)0), lap = conv;
+    for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
+        const int j = L.id[e];
+        const V4<T> G = L.geo[e];
+        const V3<T> g = gvec(G);
+        const V3<T> vj = j >= 0 ? ld3(P.vel, j) : ghost_vec<T, kFly>(B, P.vel, B.vel, -1 - j);
+        const V3<T> dv = vj - vi;
+        conv = conv + (G.w * dot(vi, g)) * dv;
+        lap = lap - G.z * dv;
+    }
+    const V3<T> acc = -conv + nu * lap;
+    st3(P.acc, i, acc);
+    st3(P.vel_star, i, vi + dt * acc);
+}
+
+// Sum of v over the workgroup (fixed order: wave trees, then waves in order).
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < nw; ++k) s += lds[k];
+    __syncthreads();
+    return s;
+}
+
+// KERNEL_EISPH_PPE.py:12-79 Kernel_build_ppe_eisph; the FP64 sum of the raw
+// bi (EISPH.py:72) into part[0] and residual[0], the raw bi[0] into part[1];
+// with kFly the ghost threads publish vel_star (KERNEL_EISPH_BC.py:6-31,
+// predicted).
+template <typename T, bool kFly>
+__global__ void __launch_bounds__(kEisThreads)
+k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part, int *done) {
+    __shared__ double lds[kEisThreads / 64];
+    __shared__ int last;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double mine = 0.0;
+    if (t < P.n) {
+        const int i = t;
+        const V3<T> vi = ld3(P.vel_star, i);
+        T div = 0, A = 0, Ap = 0;
+        for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
+            const int j = L.id[e];
+            const V4<T> G = L.geo[e];
+            const V3<T> g = gvec(G);
+            V3<T> vj;
+            T pj;
+            if (j >= 0) {
+                vj = ld3(P.vel_star, j);
+                pj = P.pres[j];
+            } else {
+                const int b = -1 - j;
+                vj = ghost_vec<T, kFly>(B, P.vel_star, B.vel_star, b);
+                pj = P.pres[B.mirror[b]];
+            }
+            div = div + G.w * dot(vj - vi, g);
+            A = A + G.z;
+            Ap = Ap + G.z * pj;
+        }
+        const T bi = rho0 * div / dt;
+        P.div[i] = div;
+        P.Aij[i] = A;
+        P.bi[i] = bi;
+        P.Aijpj[i] = Ap;
+        mine = (double)bi;
+        if (i == 0) part[1] = (double)bi;
+    } else if (kFly && t - P.n < B.n) {
+        const int b = t - P.n;
+        st3(B.vel_star, b, (T)2.0 * ld3(B.vel_bc, b) - ld3(P.vel_star, B.mirror[b]));
+    }
+    const double s = block_sum(mine, lds);
+    if (threadIdx.x == 0) {
+        part[2 + blockIdx.x] = s;
+        __threadfence();
+        last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    // Last workgroup: every other workgroup's sum is visible (released by its
+    // fence before the counter).  Add them in workgroup order.
+    __threadfence();
+    double acc = 0.0;
+    const int nb = gridDim.x;
+    const int per = (nb + blockDim.x - 1) / blockDim.x;
+    const int a = threadIdx.x * per, z = min(nb, a + per);
+    for (int k = a; k < z; ++k)
+        acc += __hip_atomic_load(&part[2 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // consecutive chunks per thread, then the fixed tree of block_sum
+    const double total = block_sum(acc, lds);
+    if (threadIdx.x == 0) {
+        part[0] = total;
+        P.res[0] = (T)total;
+        *done = 0;
+    }
+}
+
+// KERNEL_EISPH_PPE.py:82-125 Kernel_remove_bi_mean and Kernel_solve_ppe_eisph.
+// residual[0] holds the sum rounded to T, as the reference's array_sum
+// leaves it; the gauge row 0 uses its own bi after the mean removal.
+template <typename T>
+__global__ void __launch_bounds__(kEisThreads)
+k_eis_solve(EisV<T> P, const double *part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const T mean = (T)part[0] / (T)P.n;
+    const T bi = P.bi[i] - mean;
+    P.bi[i] = bi;
+    T gauge = 0;
+    const T gA = P.Aij[0];
+    if (mabs(gA) > (T)1.0e-12) gauge = ((T)part[1] - mean + P.Aijpj[0]) / gA;
+    T p = P.pres[i];
+    const T A = P.Aij[i];
+    if (mabs(A) > (T)1.0e-12) p = (bi + P.Aijpj[i]) / A;
+    P.pres[i] = p - gauge;
+}
+
+// KERNEL_EISPH_step.py:12-61 Kernel_correct_eisph; with kFly the ghost
+// threads publish pres (KERNEL_EISPH_BC.py:34-47).
+template <typename T, bool kFly>
+__global__ void __launch_bounds__(kEisThreads)
+k_eis_correct(EisV<T> P, EisV<T> B, NbrV<T> L, T dt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) {
+        const int b = t - P.n;
+        if (kFly && b < B.n) B.pres[b] = P.pres[B.mirror[b]];
+        return;
+    }
+    const int i = t;
+    const T pi = P.pres[i];
+    V3<T> gp = mk3((T)0, (T)0, (T)0);
+    for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
+        const int j = L.id[e];
+        const V4<T> G = L.geo[e];
+        T pj;
+        if (j >= 0) pj = P.pres[j];
+        else pj = kFly ? P.pres[B.mirror[-1 - j]] : B.pres[-1 - j];
+        gp = gp + (G.w * (pj - pi)) * gvec(G);
+    }
+    const V3<T> old = ld3(P.vel, i);
+    const V3<T> v = ld3(P.vel_star, i) - (dt * gp) / P.rho[i];
+    st3(P.vel, i, v);
+    st3(P.acc, i, (v - old) / dt);
+}
+
+// KERNEL_EISPH_step.py:64-121 Kernel_diagnostics_eisph (kDiag); with kFly
+// the ghost threads publish vel (KERNEL_EISPH_BC.py:6-31).
+template <typename T, bool kFly, bool kDiag>
+__global__ void __launch_bounds__(kEisThreads)
+k_eis_finish(EisV<T> P, EisV<T> B, NbrV<T> L) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) {
+        const int b = t - P.n;
+        if (kFly && b < B.n)
+            st3(B.vel, b, (T)2.0 * ld3(B.vel_bc, b) - ld3(P.vel, B.mirror[b]));
+        return;
+    }
+    if (!kDiag) return;
+    const int i = t;
+    const V3<T> vi = ld3(P.vel, i);
+    const T pi = P.pres[i];
+    T div = 0, op = 0;
+    for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
+        const int j = L.id[e];
+        const V4<T> G = L.geo[e];
+        V3<T> vj;
+        T pj;
+        if (j >= 0) {
+            vj = ld3(P.vel, j);
+            pj = P.pres[j];
+        } else {
+            const int b = -1 - j;
+            vj = ghost_vec<T, kFly>(B, P.vel, B.vel, b);
+            pj = kFly ? P.pres[B.mirror[b]] : B.pres[b];
+        }
+        div = div + G.w * dot(vj - vi, gvec(G));
+        op = op + G.z * (pi - pj);
+    }
+    P.div[i] = div;
+    P.res[i] = op - P.bi[i];
+}
+
+template <typename T>
+__global__ void k_eis_dirichlet(EisV<T> P, EisV<T> B, int predicted) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B.n) return;
+    const T *src = predicted ? P.vel_star : P.vel;
+    T *dst = predicted ? B.vel_star : B.vel;
+    st3(dst, b, (T)2.0 * ld3(B.vel_bc, b) - ld3(src, B.mirror[b]));
+}
+
+template <typename T>
+__global__ void k_eis_pressure_boundary(EisV<T> P, EisV<T> B) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B.n) B.pres[b] = P.pres[B.mirror[b]];
+}
+
+// ---------------------------------------------------------------- checks
+int check_points(const sphdem_eisph_t *S, const char *what, bool fluid) {
+    SPHDEM_REQUIRE(S, std::string(what) + " is NULL");
+    SPHDEM_REQUIRE(S->n >= 0, std::string(what) + ": negative count");
+    if (S->n == 0) return SPHDEM_OK;
+    SPHDEM_REQUIRE(S->pos && S->vel && S->vel_star && S->vel_bc && S->rho && S->m && S->pres &&
+                       S->mirror,
+                   std::string(what) + ": field pointer is NULL");
+    if (fluid)
+        SPHDEM_REQUIRE(S->acc && S->kgc && S->Aij && S->bi && S->Aijpj && S->divergence &&
+                           S->residual,
+                       std::string(what) + ": field pointer is NULL");
+    return SPHDEM_OK;
+}
+
+int check_pair(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P, const sphdem_eisph_t *B) {
+    SPHDEM_REQUIRE(p, "params is NULL");
+    SPHDEM_REQUIRE(p->precision == 4 || p->precision == 8, "precision must be 4 or 8");
+    int rc = check_points(P, "fluid points", true);
+    if (rc) return rc;
+    return check_points(B, "ghost points", false);
+}
+
+int check_grids(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P, const sphdem_eisph_t *B,
+                const sphdem_grid_t *gs, const sphdem_grid_t *gb) {
+    SPHDEM_REQUIRE(gs && gb, "grid is NULL");
+    SPHDEM_REQUIRE(gs->precision == p->precision && gb->precision == p->precision,
+                   "grid precision differs from the point precision");
+    SPHDEM_REQUIRE(gs->n == P->n, "fluid grid was built from a different number of points");
+    SPHDEM_REQUIRE(gb->n == B->n, "ghost grid was built from a different number of points");
+    SPHDEM_REQUIRE(gs->n_inst <= 1 && gb->n_inst <= 1, "batched grids are not supported here");
+    SPHDEM_REQUIRE(gs->n == 0 || (gs->point_ids && gs->sorted), "fluid grid is not built");
+    SPHDEM_REQUIRE(gb->n == 0 || (gb->point_ids && gb->sorted), "ghost grid is not built");
+    return SPHDEM_OK;
+}
+
+int check_list(const sphdem_eisph_t *P, const sphdem_eisph_nbr_t *L) {
+    SPHDEM_REQUIRE(L, "neighbour list is NULL");
+    SPHDEM_REQUIRE(L->n == P->n, "neighbour list was built for a different number of points");
+    SPHDEM_REQUIRE(L->offset && (L->cap == 0 || (L->id && L->geo)), "neighbour list is not built");
+    return SPHDEM_OK;
+}
+
+int check_sums(const sphdem_eisph_nbr_t *L) {
+    SPHDEM_REQUIRE(L->part && L->done, "neighbour list has no sum buffers");
+    return SPHDEM_OK;
+}
+
+#define EIS_DISPATCH(prec, CALL)       \
+    do {                               \
+        if ((prec) == 4) {             \
+            using T = float;           \
+            CALL;                      \
+        } else {                       \
+            using T = double;          \
+            CALL;                      \
+        }                              \
+    } while (0)
+
+inline dim3 fluid_grid(int n) { return dim3(blocks_for(n, kEisThreads)); }
+inline dim3 all_grid(int nf, int nb) { return dim3(blocks_for((long)nf + nb, kEisThreads)); }
+
+template <typename T, bool kFly>
+void launch_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, const sphdem_eisph_t *B,
+                const sphdem_eisph_nbr_t *L, hipStream_t st) {
+    const int nb_ghost = kFly ? B->n : 0;
+    hipLaunchKernelGGL((k_eis_ppe<T, kFly>), all_grid(P->n, nb_ghost), dim3(kEisThreads), 0, st,
+                       eview<T>(P), eview<T>(B), nview<T>(L), (T)p->rho0, (T)p->dt, L->part,
+                       L->done);
+}
+
+}  // namespace
+}  // namespace sphdem
+
+using namespace sphdem;
+
+extern "C" {
+
+long sphdem_eisph_part_size(int n_fluid, int n_ghost) {
+    if (n_fluid < 0 || n_ghost < 0) return -1;
+    return 2 + (long)blocks_for((long)n_fluid + n_ghost, kEisThreads);
+}
+
+int sphdem_eisph_kgc(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, const sphdem_eisph_t *B,
+                     const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_grids(p, P, B, gs, gb);
+    if (rc) return rc;
+    if (P->n == 0) return SPHDEM_OK;
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL(k_eis_kgc<T>, fluid_grid(P->n), dim3(kEisThreads), 0,
+                                    as_stream(stream), eview<T>(P), eview<T>(B), view<T>(gs),
+                                    view<T>(gb), (T)p->support, (T)p->h));
+    return check_launch("eisph_kgc");
+}
+
+int sphdem_eisph_nbr_count(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P,
+                           const sphdem_eisph_t *B, const sphdem_grid_t *gs,
+                           const sphdem_grid_t *gb, sphdem_eisph_nbr_t *L, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_grids(p, P, B, gs, gb);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(L && L->offset && L->count && L->scan_tmp, "neighbour list buffers are NULL");
+    SPHDEM_REQUIRE(L->n == P->n, "neighbour list was sized for a different number of points");
+    hipStream_t st = as_stream(stream);
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL(k_eis_count<T>, dim3(blocks_for(P->n + 1, kEisThreads)),
+                                    dim3(kEisThreads), 0, st, eview<T>(P), view<T>(gs),
+                                    view<T>(gb), (T)p->support, L->count));
+    rc = check_launch("eisph_nbr_count");
+    if (rc) return rc;
+    return exclusive_scan(L->count, L->offset, P->n + 1, L->scan_tmp, st);
+}
+
+int sphdem_eisph_nbr_fill(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P,
+                          const sphdem_eisph_t *B, const sphdem_grid_t *gs,
+                          const sphdem_grid_t *gb, sphdem_eisph_nbr_t *L, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_grids(p, P, B, gs, gb);
+    if (!rc) rc = check_list(P, L);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(L->cap >= 0, "negative neighbour list capacity");
+    if (P->n == 0) return SPHDEM_OK;
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL(k_eis_fill<T>, fluid_grid(P->n), dim3(kEisThreads), 0,
+                                    as_stream(stream), eview<T>(P), eview<T>(B), view<T>(gs),
+                                    view<T>(gb), (T)p->support, (T)p->h, L->offset, L->id,
+                                    (V4<T> *)L->geo, L->cap));
+    return check_launch("eisph_nbr_fill");
+}
+
+int sphdem_eisph_dirichlet(int precision, const sphdem_eisph_t *P, sphdem_eisph_t *B,
+                           int predicted, void *stream) {
+    SPHDEM_REQUIRE(precision == 4 || precision == 8, "precision must be 4 or 8");
+    int rc = check_points(P, "fluid points", false);
+    if (!rc) rc = check_points(B, "ghost points", false);
+    if (rc) return rc;
+    if (B->n == 0) return SPHDEM_OK;
+    EIS_DISPATCH(precision,
+                 hipLaunchKernelGGL(k_eis_dirichlet<T>, dim3(blocks_for(B->n, kEisThreads)),
+                                    dim3(kEisThreads), 0, as_stream(stream), eview<T>(P),
+                                    eview<T>(B), predicted));
+    return check_launch("eisph_dirichlet");
+}
+
+int sphdem_eisph_pressure_boundary(int precision, const sphdem_eisph_t *P, sphdem_eisph_t *B,
+                                   void *stream) {
+    SPHDEM_REQUIRE(precision == 4 || precision == 8, "precision must be 4 or 8");
+    int rc = check_points(P, "fluid points", false);
+    if (!rc) rc = check_points(B, "ghost points", false);
+    if (rc) return rc;
+    if (B->n == 0) return SPHDEM_OK;
+    EIS_DISPATCH(precision,
+                 hipLaunchKernelGGL(k_eis_pressure_boundary<T>, dim3(blocks_for(B->n, kEisThreads)),
+                                    dim3(kEisThreads), 0, as_stream(stream), eview<T>(P),
+                                    eview<T>(B)));
+    return check_launch("eisph_pressure_boundary");
+}
+
+int sphdem_eisph_advec_vis(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                           const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_list(P, L);
+    if (rc) return rc;
+    if (P->n == 0) return SPHDEM_OK;
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL((k_eis_advec<T, false>), fluid_grid(P->n), dim3(kEisThreads),
+                                    0, as_stream(stream), eview<T>(P), eview<T>(B), nview<T>(L),
+                                    (T)p->nu, (T)p->dt));
+    return check_launch("eisph_advec_vis");
+}
+
+int sphdem_eisph_build_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                           const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_list(P, L);
+    if (!rc) rc = check_sums(L);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(P->n > 0, "the pressure equation needs fluid points");
+    EIS_DISPATCH(p->precision, (launch_ppe<T, false>(p, P, B, L, as_stream(stream))));
+    return check_launch("eisph_build_ppe");
+}
+
+int sphdem_eisph_solve_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                           const sphdem_eisph_nbr_t *L, void *stream) {
+    SPHDEM_REQUIRE(p && (p->precision == 4 || p->precision == 8), "invalid params");
+    int rc = check_points(P, "fluid points", true);
+    if (!rc) rc = check_list(P, L);
+    if (!rc) rc = check_sums(L);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(P->n > 0, "the pressure equation needs fluid points");
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL(k_eis_solve<T>, fluid_grid(P->n), dim3(kEisThreads), 0,
+                                    as_stream(stream), eview<T>(P), (const double *)L->part));
+    return check_launch("eisph_solve_ppe");
+}
+
+int sphdem_eisph_correct(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                         const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_list(P, L);
+    if (rc) return rc;
+    if (P->n == 0) return SPHDEM_OK;
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL((k_eis_correct<T, false>), fluid_grid(P->n),
+                                    dim3(kEisThreads), 0, as_stream(stream), eview<T>(P),
+                                    eview<T>(B), nview<T>(L), (T)p->dt));
+    return check_launch("eisph_correct");
+}
+
+int sphdem_eisph_diagnostics(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
+                             const sphdem_eisph_t *B, const sphdem_eisph_nbr_t *L, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_list(P, L);
+    if (rc) return rc;
+    if (P->n == 0) return SPHDEM_OK;
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL((k_eis_finish<T, false, true>), fluid_grid(P->n),
+                                    dim3(kEisThreads), 0, as_stream(stream), eview<T>(P),
+                                    eview<T>(B), nview<T>(L)));
+    return check_launch("eisph_diagnostics");
+}
+
+int sphdem_eisph_step(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem_eisph_t *B,
+                      const sphdem_eisph_nbr_t *L, int diagnostics, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_list(P, L);
+    if (!rc) rc = check_sums(L);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(P->n > 0, "the pressure equation needs fluid points");
+    hipStream_t st = as_stream(stream);
+    const dim3 blk(kEisThreads), gf = fluid_grid(P->n), ga = all_grid(P->n, B->n);
+    EIS_DISPATCH(p->precision, ({
+        const EisV<T> Pv = eview<T>(P), Bv = eview<T>(B);
+        const NbrV<T> Lv = nview<T>(L);
+        hipLaunchKernelGGL((k_eis_advec<T, true>), gf, blk, 0, st, Pv, Bv, Lv, (T)p->nu,
+                           (T)p->dt);
+        launch_ppe<T, true>(p, P, B, L, st);
+        hipLaunchKernelGGL(k_eis_solve<T>, gf, blk, 0, st, Pv, (const double *)L->part);
+        hipLaunchKernelGGL((k_eis_correct<T, true>), ga, blk, 0, st, Pv, Bv, Lv, (T)p->dt);
+        if (diagnostics)
+            hipLaunchKernelGGL((k_eis_finish<T, true, true>), ga, blk, 0, st, Pv, Bv, Lv);
+        else
+            hipLaunchKernelGGL((k_eis_finish<T, true, false>), ga, blk, 0, st, Pv, Bv, Lv);
+    }));
+    return check_launch("eisph_step");
+}
+
+}  // extern "C"

@@ -144,3 +144,57 @@ def save_gif(frames: list, solv, field: str = "pres", fps: int = 20, clip: float
                                                                    writer=PillowWriter(fps=fps))
     plt.close(fig)
     return path
+
+
+def save_cavity_gif(positions, velocity_frames: list, times: list, solv, path=None) -> str:
+    """GIF of the Eulerian cavity (output/gif_eisph.py:11-61): fixed x-z
+    points coloured by speed |(u, w)| [m/s] on one scale for all frames,
+    velocity arrows on a sample of about 150 points, bounds from the point
+    cloud.  Writes `path` (default solv.animation_path); returns its
+    absolute path."""
+    if len(velocity_frames) != len(times) or not velocity_frames:
+        raise ValueError("velocity_frames and times must have the same non-zero length")
+    positions = np.asarray(positions)
+    if positions.ndim != 2 or positions.shape[1] != 3 or not len(positions):
+        raise ValueError("positions must be a nonempty [N,3] array")
+    if not np.isfinite(positions).all():
+        raise ValueError("positions must be finite")
+    frames = [np.asarray(v) for v in velocity_frames]
+    for v in frames:
+        if v.shape != positions.shape or not np.isfinite(v).all():
+            raise ValueError("every velocity frame must be finite and match positions")
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    from PIL import Image
+
+    dest = os.path.abspath(solv.animation_path if path is None else str(path))
+    os.makedirs(os.path.dirname(dest), exist_ok=True)
+    speed = [np.linalg.norm(v[:, (0, 2)], axis=1) for v in frames]
+    vmax = max(max(float(s.max()) for s in speed), 1e-8)
+    x, z = positions[:, 0], positions[:, 2]
+    span = max(float(np.ptp(x)), float(np.ptp(z)), solv.h)
+    pad = 0.04 * span
+    pick = slice(None, None, max(1, len(positions) // 150))
+    fig, ax = plt.subplots(figsize=(6.1, 5.5), dpi=100)
+    dots = ax.scatter(x, z, c=speed[0], cmap="turbo", vmin=0.0, vmax=vmax, marker="s",
+                      s=max(3.0, min(70.0, 25000.0 / len(positions))))
+    arrows = ax.quiver(x[pick], z[pick], frames[0][pick, 0], frames[0][pick, 2], color="white",
+                       angles="xy", scale_units="xy", scale=15.0 * vmax / span, width=0.003)
+    ax.set(xlabel="x [m]", ylabel="z [m]", aspect="equal",
+           xlim=(x.min() - pad, x.max() + pad), ylim=(z.min() - pad, z.max() + pad))
+    title = ax.set_title(f"Eulerian ISPH   t = {times[0]:.3f} s")
+    fig.colorbar(dots, ax=ax, pad=0.03).set_label("speed [m/s]")
+    fig.tight_layout()
+    images = []
+    for s, v, t in zip(speed, frames, times):
+        dots.set_array(s)
+        arrows.set_UVC(v[pick, 0], v[pick, 2])
+        title.set_text(f"Eulerian ISPH   t = {t:.3f} s")
+        fig.canvas.draw()
+        rgba = np.asarray(fig.canvas.buffer_rgba())
+        images.append(Image.fromarray(rgba[..., :3].copy()).convert("P", palette=Image.ADAPTIVE))
+    plt.close(fig)
+    images[0].save(dest, save_all=True, append_images=images[1:], loop=0,
+                   duration=int(round(1000.0 / solv.gif_fps)))
+    return dest

@@ -50,6 +50,7 @@ class HashGrid:
         self.dtype = torch.float32
         self.points = None
         self._w_tag = None
+        self.builds = 0  # number of builds so far (caches derived from the buckets key on it)
 
     def _alloc_buckets(self) -> None:
         ncell = self.dim_x * self.dim_y * self.dim_z * self.n_inst
@@ -131,6 +132,7 @@ class HashGrid:
         _lib.call("sphdem_grid_build", g, _lib.ptr(points, self.dtype, "points"), n, radius,
                   _lib.ptr(weights, self.dtype, "weights"), _lib.stream_handle())
         self._w_tag = _tag(weights)
+        self.builds += 1
         return self
 
     def ensure_weights(self, w: torch.Tensor) -> None:

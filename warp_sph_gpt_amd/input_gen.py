@@ -2,8 +2,8 @@
 
 Mirrors input_gen/config.py:10-363 (GenerationConfig, CoupledGenerationConfig),
 input_gen/gen_ptl.py:8-101 (DamPtlGeneration), input_gen/gen_dem.py:9-125
-(DEMPtlGeneration) and input_gen/generate.py:20-44 (write_particle_file,
-generate_files).  The lattice formulas, float32 rounding and the "%.9g"
+(DEMPtlGeneration), input_gen/gen_eisph.py:8-114 (CavityPtlGeneration) and
+input_gen/generate.py:20-57 (write_particle_file, generate_files, main).  The lattice formulas, float32 rounding and the "%.9g"
 tab-separated format are kept, so the reference's shipped input files are
 reproduced byte for byte.  Seeded random sequential addition of DEM
 spheres (`rsa_spheres`) is an addition for the larger workloads.
@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from .input_reader import INPUT_FIELDS, column_indices
-from .struct import BNDptl, DEMBNDptl, DEMptl, SPHptl
+from .struct import BNDptl, DEMBNDptl, DEMptl, EISPHptl, SPHptl
 
 
 @dataclass
@@ -80,6 +80,11 @@ class GenerationConfig:
     dem_porosity_min: float = 0.05
     dem_porosity_max: float = 1.0
     dem_dt_safety: float = 0.1
+    length: float = 1.0
+    lid_velocity: float = 1.0
+    reynolds: float = 100.0
+    gif_fps: int = 20
+    animation_path: str = "animation/lid_driven_cavity.gif"
 
     def __post_init__(self) -> None:
         if self.h is None:
@@ -114,6 +119,42 @@ class GenerationConfig:
     @property
     def dem_support(self) -> float:
         return 2.0 * self.dem_h
+
+    @property
+    def nu(self) -> float:
+        """Cavity kinematic viscosity U_lid L / Re (input_gen/config.py:220-223)."""
+        return self.lid_velocity * self.length / self.reynolds
+
+    @property
+    def cells(self) -> int:
+        """Cell-centred fluid points along one cavity side."""
+        return int(round(self.length / self.dx))
+
+    def validate_eisph(self) -> None:
+        """input_gen/config.py:269-300: cavity lattice, ghost layers covering
+        the 2h support, and the explicit advection dt limit."""
+        for name in ("length", "dx", "h_factor", "h", "support", "rho0",
+                     "lid_velocity", "reynolds", "dt"):
+            value = getattr(self, name)
+            if not math.isfinite(value) or value <= 0.0:
+                raise ValueError(f"{name} must be finite and positive")
+        for name in ("bnd_layer", "n_steps", "output_step", "gif_fps", "grid_slice"):
+            value = getattr(self, name)
+            if not isinstance(value, int) or value < 1:
+                raise ValueError(f"{name} must be a positive integer")
+        if not math.isclose(self.h, self.h_factor * self.dx):
+            raise ValueError("EISPH smoothing length must equal h_factor*dx")
+        if not math.isclose(self.support, 2.0 * self.h):
+            raise ValueError("Wendland support must equal 2*h")
+        if self.cells < 5 or not math.isclose(self.cells * self.dx, self.length,
+                                               rel_tol=0.0, abs_tol=1.0e-10):
+            raise ValueError("length must be an integer multiple of dx with at least 5 cells")
+        if self.bnd_layer < math.ceil(self.support / self.dx):
+            raise ValueError("bnd_layer must cover the complete kernel support")
+        if self.bnd_layer > self.cells:
+            raise ValueError("bnd_layer cannot exceed the cavity cell count")
+        if self.dt > 0.25 * self.dx / self.lid_velocity:
+            raise ValueError("dt exceeds the explicit advection limit")
 
     def validate_sph(self) -> None:
         """input_gen/config.py:227-263: domain, lattice and dt checks."""
@@ -320,6 +361,68 @@ class DEMPtlGeneration:
         return D.to(s.device), W.to(s.device)
 
 
+class CavityPtlGeneration:
+    """Fixed points of the lid-driven cavity (input_gen/gen_eisph.py:8-114).
+
+    The square cavity lies in the x-z plane: cell-centred fluid points, and
+    ghost points continuing the lattice bnd_layer cells beyond the four
+    walls.  Each ghost reflects through its nearest wall onto a fluid row
+    (`mirror`); ghosts above the cavity (the lid) prescribe u = lid_velocity,
+    the others no slip."""
+
+    def __init__(self, solv: GenerationConfig) -> None:
+        self.solv = solv
+
+    def fluid_particle(self) -> np.ndarray:
+        """Cell-centred fluid positions [cells^2, 3], x fastest."""
+        n = self.solv.cells
+        axis = (np.arange(n, dtype=np.float32) + 0.5) * self.solv.dx
+        gx, gz = np.meshgrid(axis, axis, indexing="xy")
+        return np.stack((gx.ravel(), np.zeros(gx.size, np.float32), gz.ravel()), axis=1)
+
+    def boundary_particle(self) -> tuple:
+        """Ghost positions [N_bnd, 3], prescribed velocity [N_bnd, 3] and
+        mirrored fluid row [N_bnd]."""
+        n, layers, dx = self.solv.cells, self.solv.bnd_layer, self.solv.dx
+        ids = np.arange(-layers, n + layers, dtype=np.int32)
+        gx, gz = np.meshgrid(ids, ids, indexing="xy")
+        shell = (gx < 0) | (gx >= n) | (gz < 0) | (gz >= n)
+        ix, iz = gx[shell], gz[shell]
+        pos = np.stack(((ix + 0.5) * dx, np.zeros(ix.size), (iz + 0.5) * dx),
+                       axis=1).astype(np.float32)
+        vel_bc = np.zeros_like(pos)
+        vel_bc[(iz >= n) & (ix >= 0) & (ix < n), 0] = self.solv.lid_velocity
+        mx = np.where(ix < 0, -ix - 1, np.where(ix >= n, 2 * n - ix - 1, ix))
+        mz = np.where(iz < 0, -iz - 1, np.where(iz >= n, 2 * n - iz - 1, iz))
+        mirror = (mz * n + mx).astype(np.int32)
+        if np.any((mirror < 0) | (mirror >= n * n)):
+            raise ValueError("bnd_layer cannot exceed the cavity cell count")
+        return pos, vel_bc, mirror
+
+    def _build(self, pos, vel, vel_bc, mirror) -> EISPHptl:
+        solv = self.solv
+        n = len(pos)
+        p = EISPHptl.allocate(n)
+        p.pos = _f32(pos)
+        p.vel = _f32(vel)
+        p.vel_star = _f32(vel)
+        p.vel_bc = _f32(vel_bc)
+        p.rho = torch.full((n,), solv.rho0, dtype=torch.float32)
+        p.m = torch.full((n,), solv.rho0 * solv.dx ** 2, dtype=torch.float32)
+        p.mirror = torch.from_numpy(np.asarray(mirror, dtype=np.int32))
+        return p.to(solv.device) if str(solv.device) != "cpu" else p
+
+    def build(self) -> tuple:
+        """(fluid, ghost) EISPHptl structs on solv.device; the fluid starts at
+        rest, so the ghost velocity starts at 2 vel_bc."""
+        self.solv.validate_eisph()
+        fluid_pos = self.fluid_particle()
+        bpos, vel_bc, mirror = self.boundary_particle()
+        zero = np.zeros_like(fluid_pos)
+        return (self._build(fluid_pos, zero, zero, np.arange(len(fluid_pos), dtype=np.int32)),
+                self._build(bpos, 2.0 * vel_bc, vel_bc, mirror))
+
+
 def check_dem_inside(pos: np.ndarray, radius, s: GenerationConfig) -> None:
     """Sphere extents inside the five closed tank sides (gen_dem.py:32-40)."""
     r = np.broadcast_to(np.asarray(radius, np.float64), (len(pos),))
@@ -415,12 +518,16 @@ def write_particle_file(path: str | Path, kind: str, particle) -> Path:
 
 
 def generate_files(output_dir: str | Path,
-                   coupled: CoupledGenerationConfig | None = None) -> dict:
-    """Write input_{SPH,BND,DEM,DEMBND}.txt of the dam-break case
-    (input_gen/generate.py:34-44, without the EISPH cavity)."""
+                   coupled: CoupledGenerationConfig | None = None,
+                   cavity: GenerationConfig | None = None) -> dict:
+    """Write the six input files: input_{SPH,BND,DEM,DEMBND}.txt of the dam
+    break and input_{EISPH,EISPHBND}.txt of the cavity
+    (input_gen/generate.py:34-44)."""
     coupled = CoupledGenerationConfig(device="cpu") if coupled is None else coupled
+    cavity = GenerationConfig(device="cpu") if cavity is None else cavity
     particles = dict(zip(("SPH", "BND"), DamPtlGeneration(coupled).build()))
     particles.update(zip(("DEM", "DEMBND"), DEMPtlGeneration(coupled).build()))
+    particles.update(zip(("EISPH", "EISPHBND"), CavityPtlGeneration(cavity).build()))
     return {kind: write_particle_file(Path(output_dir) / f"input_{kind}.txt", kind, p)
             for kind, p in particles.items()}
 

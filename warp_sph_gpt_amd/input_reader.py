@@ -1,10 +1,10 @@
 """Particle input files: numeric-header column-ID text format.
 
-Mirrors input/input_reader.py:1-184: the column IDs of each kind, the
+Mirrors input/input_reader.py:1-220: the column IDs of each kind, the
 validation and error messages of `_read_table`, `input_parser`,
-`validate_dem_particles`, `dem_search_scales` and `load_sph_particles`.
-Structs are returned as torch tensors on the requested device.  The EISPH
-kinds of the reference belong to a different solver and are not read here.
+`validate_dem_particles`, `dem_search_scales`, `load_sph_particles`,
+`validate_eisph_particles` and `load_eisph_particles`.  Structs are returned
+as torch tensors on the requested device.
 """
 from __future__ import annotations
 
@@ -13,7 +13,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
-from .struct import BNDptl, DEMBNDptl, DEMptl, SPHptl
+from .struct import BNDptl, DEMBNDptl, DEMptl, EISPHptl, SPHptl
 
 INPUT_FIELDS = {
     "SPH": (("pos", 3), ("vel", 3), ("rho", 1), ("m", 1)),
@@ -22,8 +22,11 @@ INPUT_FIELDS = {
             ("inertia", 1), ("K", 1), ("eta", 1), ("mu", 1), ("h", 1)),
     "DEMBND": (("pos", 3), ("vel", 3), ("omega", 3), ("radius", 1), ("rho", 1), ("m", 1),
                ("inertia", 1), ("K", 1), ("eta", 1), ("mu", 1)),
+    "EISPH": (("pos", 3), ("vel", 3), ("rho", 1), ("m", 1)),
+    "EISPHBND": (("pos", 3), ("vel_bc", 3), ("rho", 1), ("m", 1), ("mirror", 1)),
 }
-STRUCTURES = {"SPH": SPHptl, "BND": BNDptl, "DEM": DEMptl, "DEMBND": DEMBNDptl}
+STRUCTURES = {"SPH": SPHptl, "BND": BNDptl, "DEM": DEMptl, "DEMBND": DEMBNDptl,
+              "EISPH": EISPHptl, "EISPHBND": EISPHptl}
 PROJECT_ROOT = Path(__file__).resolve().parents[1]
 _F32_MAX = float(np.finfo(np.float32).max)
 
@@ -76,6 +79,10 @@ def _read_table(path: Path, kind: str) -> dict:
     for field in ("eta", "mu"):
         if field in result and np.any(result[field] < 0.0):
             raise ValueError(f"{path}: {field} must be nonnegative")
+    if "mirror" in result:
+        ids = result["mirror"]
+        if np.any((ids != np.floor(ids)) | (ids < 0) | (ids > np.iinfo(np.int32).max)):
+            raise ValueError(f"{path}: mirror must be a nonnegative int32 fluid row index")
     if "radius" in result:
         volume = (4.0 / 3.0) * np.pi * result["radius"] ** 3
         if not np.allclose(result["m"], result["rho"] * volume, rtol=1e-5, atol=0.0):
@@ -91,7 +98,8 @@ def input_parser(file_path, kind: str | None = None, device="cpu"):
 
     Computed fields start at zero, rho_raw copies rho, fluid and DEM porosity
     start at one, DEM volume is (4/3)*pi*radius^3, and both contact CSR sides
-    are empty with zero offsets."""
+    are empty with zero offsets.  EISPH fluid rows get vel_star = vel and the
+    identity mirror; ghost mirror rows are read as int32."""
     path = Path(file_path)
     kind = path.stem.removeprefix("input_") if kind is None else kind
     if kind not in STRUCTURES:
@@ -100,7 +108,13 @@ def input_parser(file_path, kind: str | None = None, device="cpu"):
     n = len(data["pos"])
     particle = STRUCTURES[kind].allocate(n, device="cpu")
     for name, values in data.items():
-        t = torch.from_numpy(np.asarray(values, np.float64).astype(np.float32))
+        if name == "mirror":
+            setattr(particle, name, torch.from_numpy(np.asarray(values).astype(np.int32)))
+            continue
+        # column selection leaves numpy in column-major order; the kernels
+        # take packed row-major (N, 3) arrays
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(values, np.float64),
+                                                  dtype=np.float32))
         setattr(particle, name, t.reshape(n, 3) if t.ndim == 2 else t)
     if kind in ("SPH", "BND"):
         particle.rho_raw = particle.rho.clone()
@@ -109,6 +123,9 @@ def input_parser(file_path, kind: str | None = None, device="cpu"):
     if kind in ("DEM", "DEMBND"):
         volume = (4.0 / 3.0) * np.pi * data["radius"] ** 3
         particle.volume = torch.from_numpy(np.asarray(volume, np.float64).astype(np.float32))
+    if kind == "EISPH":
+        particle.vel_star = particle.vel.clone()
+        particle.mirror = torch.arange(n, dtype=torch.int32)
     return particle.to(device) if str(device) != "cpu" else particle
 
 
@@ -158,3 +175,44 @@ def load_sph_particles(solv):
                         for k in ("DEM", "DEMBND"))
         validate_dem_particles(solv, dem, dem_bnd)
     return sph, bnd, dem, dem_bnd
+
+
+def validate_eisph_particles(solv, fluid, boundary) -> None:
+    """A fixed planar EISPH cloud and its mirror map (input_reader.py:187-208):
+    nonempty sets, mirror rows inside the fluid file, uniform rho = rho0,
+    points in the x-z plane, and the explicit predictor's advection and
+    viscosity dt limits for the loaded speeds and spacing."""
+    solv.validate_eisph()
+    n = fluid.pos.shape[0]
+    if n == 0 or boundary.pos.shape[0] == 0:
+        raise ValueError("EISPH needs nonempty fluid and ghost files")
+    if np.any(boundary.mirror.detach().cpu().numpy() >= n):
+        raise ValueError("EISPHBND mirror is outside the input_EISPH fluid row range")
+    for particle in (fluid, boundary):
+        if not np.allclose(particle.rho.detach().cpu().numpy(), solv.rho0, rtol=1e-6, atol=0):
+            raise ValueError("EISPH currently requires uniform rho equal to rho0")
+        if not np.allclose(particle.pos.detach().cpu().numpy()[:, 1], 0.0, atol=1e-8):
+            raise ValueError("EISPH points must lie in the x-z plane (y=0)")
+    m = fluid.m.detach().cpu().numpy()
+    rho = fluid.rho.detach().cpu().numpy()
+    dx = float(np.sqrt(m / rho).min())
+    speed = max(float(np.linalg.norm(fluid.vel.detach().cpu().numpy(), axis=1).max()),
+                float(np.linalg.norm(boundary.vel_bc.detach().cpu().numpy(), axis=1).max()))
+    if speed > 0 and solv.dt > 0.25 * dx / speed:
+        raise ValueError("dt exceeds the loaded EISPH initial advection limit")
+    if solv.nu > 0 and solv.dt > 0.125 * dx * dx / solv.nu:
+        raise ValueError("dt exceeds the loaded EISPH viscosity limit")
+
+
+def load_eisph_particles(solv):
+    """input_EISPH.txt and input_EISPHBND.txt, validated, with the initial
+    ghost velocity 2 vel_bc - vel[mirror] in vel and vel_star
+    (input_reader.py:211-220)."""
+    directory = input_directory(solv)
+    fluid, boundary = (input_parser(directory / f"input_{k}.txt", k, solv.device)
+                       for k in ("EISPH", "EISPHBND"))
+    validate_eisph_particles(solv, fluid, boundary)
+    mirror = boundary.mirror.long()
+    boundary.vel = 2.0 * boundary.vel_bc - fluid.vel[mirror]
+    boundary.vel_star = boundary.vel.clone()
+    return fluid, boundary

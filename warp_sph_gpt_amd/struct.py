@@ -1,8 +1,10 @@
 """Particle containers: structure-of-arrays with the reference's field names.
 
-Mirrors input/struct.py:3-104 (SPHptl, BNDptl, DEMptl, DEMBNDptl).  Each
-field is one torch tensor: vec3 fields are (N, 3), scalars (N,), contact
-offsets int32 (N+1,), contact ids int32 (E,), tangential histories (E, 3).
+Mirrors input/struct.py:3-104 (SPHptl, BNDptl, DEMptl, DEMBNDptl) and
+input/struct_eisph.py:4-28 (EISPHptl).  Each field is one torch tensor: vec3
+fields are (N, 3), scalars (N,), 2x2 matrices (N, 2, 2) row-major, point
+indices int32 (N,), contact offsets int32 (N+1,), contact ids int32 (E,),
+tangential histories (E, 3).
 The tensors are plain buffers; the HIP kernels read and write them in place
 through raw device pointers.
 """
@@ -11,10 +13,11 @@ from __future__ import annotations
 import torch
 
 VEC3, SCALAR, OFFSET, IDS, TANG = "vec3", "scalar", "offset", "ids", "tang"
+MAT22, INDEX = "mat22", "index"
 
 
 class ParticleStruct:
-    """Base of the four particle kinds; FIELDS lists (name, layout)."""
+    """Base of the particle kinds; FIELDS lists (name, layout)."""
 
     FIELDS: tuple = ()
 
@@ -35,6 +38,10 @@ class ParticleStruct:
                 values[name] = torch.zeros((n, 3), dtype=dtype, device=device)
             elif layout == SCALAR:
                 values[name] = torch.zeros(n, dtype=dtype, device=device)
+            elif layout == MAT22:
+                values[name] = torch.zeros((n, 2, 2), dtype=dtype, device=device)
+            elif layout == INDEX:
+                values[name] = torch.zeros(n, dtype=torch.int32, device=device)
             elif layout == OFFSET:
                 values[name] = torch.zeros(n + 1, dtype=torch.int32, device=device)
             elif layout == IDS:
@@ -115,3 +122,16 @@ class DEMBNDptl(ParticleStruct):
     FIELDS = (("pos", VEC3), ("vel", VEC3), ("omega", VEC3), ("radius", SCALAR),
               ("rho", SCALAR), ("m", SCALAR), ("volume", SCALAR), ("inertia", SCALAR),
               ("K", SCALAR), ("eta", SCALAR), ("mu", SCALAR))
+
+
+class EISPHptl(ParticleStruct):
+    """Fixed Eulerian ISPH points, fluid or ghost (input/struct_eisph.py:4-28).
+
+    One layout for both sets: `mirror` is the stable fluid row a ghost
+    reflects (an identity map on the fluid set), `residual[0]` doubles as the
+    right-hand-side sum after the pressure assembly, as in the reference."""
+
+    FIELDS = (("pos", VEC3), ("vel", VEC3), ("vel_star", VEC3), ("vel_bc", VEC3),
+              ("acc", VEC3), ("kgc", MAT22), ("rho", SCALAR), ("m", SCALAR),
+              ("pres", SCALAR), ("Aij", SCALAR), ("bi", SCALAR), ("Aijpj", SCALAR),
+              ("divergence", SCALAR), ("residual", SCALAR), ("mirror", INDEX))
