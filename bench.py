@@ -2,6 +2,9 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
 
+(--workload e0|e1 measures the Eulerian lid-driven cavity instead: point-
+steps/s of EISPH_OneStep, see bench_cavity.)
+
 Workload (default c2): 1,562,500 fluid particles (dx = 0.004), 1,080,816
 dummy-wall particles, 2,000 DEM spheres, 14,688 DEM-wall spheres,
 dt = 2.5e-5 s, synthetic initial state from the dam-break generator.  A
@@ -56,7 +59,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: workload's)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=("c0", "c1", "c2", "c3"))
+    ap.add_argument("--workload", default="c2", choices=("c0", "c1", "c2", "c3", "e0", "e1"))
     ap.add_argument("--graph", action="store_true",
                     help="replay captured hipGraphs (always on for the c3 ensemble)")
     ap.add_argument("--instances", type=int, default=64, help="c3: number of instances")
@@ -100,6 +103,8 @@ def main(argv=None) -> int:
 
     if args.workload == "c3":
         return bench_ensemble(args)
+    if args.workload in ("e0", "e1"):
+        return bench_cavity(args)
 
     dev = torch.device("cuda:0")
     t0 = time.time()
@@ -266,6 +271,138 @@ def surrogate_rollout(sim_time: float) -> dict:
         / r["final_plain"]["front"]
     log(f"[bench] surrogate: {json.dumps(out)}")
     return out
+
+
+# Algorithmic bytes per launch of the cavity passes: (per fluid point, per
+# neighbour-list entry, per ghost).  A point's own fields (row length and
+# slice offset included) and each list entry (id 4 B + (gx, gz, A_ij, V_j)
+# 16 B) are counted once; neighbour gathers re-read arrays already counted.
+EISPH_PASS_BYTES = {
+    "eisph_advec": (4 + 12 + 24, 20, 16),          # offset, vel | acc, vel_star; vel_bc, mirror
+    "eisph_ppe": (4 + 12 + 4 + 16, 20, 28),         # offset, vel_star, pres | div, A, b, Ap
+    "eisph_solve": (16 + 8, 0, 0),                  # b, Ap, A, p | b, p
+    "eisph_correct": (4 + 4 + 12 + 4 + 12 + 24, 20, 8),  # offset, p, vel*, rho, vel | vel, acc
+    "eisph_finish": (0, 0, 28),                     # ghosts: vel_bc, mirror | vel
+}
+
+
+def bench_cavity(args) -> int:
+    """The Eulerian lid-driven cavity (source/EISPH.py): value = fluid points
+    x K / wall seconds of K steps (diagnostics off, as between output steps
+    of run_eisph), fixed grids, KGC and neighbour list built before timing."""
+    from warp_sph_gpt_amd import _lib
+    from warp_sph_gpt_amd import eisph as G
+    from warp_sph_gpt_amd.workloads import CAVITIES, cavity_case
+
+    case = CAVITIES[args.workload]
+    t0 = time.time()
+    gen, solv, P, B = cavity_case(case.name)
+    dims = None if case.cells <= 64 else (512, 1, 512)
+    gs, gb = G.prepare(solv, P, B, dims=dims)
+    L = G.neighbours(P, B, gs, gb, solv.support, solv.h)
+    steps = args.steps if args.steps is not None else case.steps
+    log(f"[bench] {case.name}: {P.n} fluid, {B.n} ghost, {L.entries} list entries; "
+        f"setup {time.time() - t0:.1f}s")
+    for _ in range(args.warmup):
+        G.EISPH_OneStep(solv, P, B, gs, gb, diagnostics=False)
+    torch.cuda.synchronize()
+    per_graph = 0
+    if args.graph:
+        per_graph = min(steps, 100)
+        steps = -(-steps // per_graph) * per_graph
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph, stream=side):
+                for _ in range(per_graph):
+                    G.EISPH_OneStep(solv, P, B, gs, gb, diagnostics=False)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done")
+
+    torch.cuda.synchronize()
+    if per_graph:
+        start = time.perf_counter()
+        for _ in range(steps // per_graph):
+            graph.replay()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - start
+        # per-pass launch times from profiled eager steps after the timed region
+        with _lib.PassTimer() as timer:
+            for _ in range(min(steps, 50)):
+                G.EISPH_OneStep(solv, P, B, gs, gb, diagnostics=False)
+            torch.cuda.synchronize()
+        timed_from = "profiled eager steps after the timed graph replays"
+    else:
+        with _lib.PassTimer() as timer:
+            start = time.perf_counter()
+            for _ in range(steps):
+                G.EISPH_OneStep(solv, P, B, gs, gb, diagnostics=False)
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - start
+        timed_from = "HIP events around every launch of the timed region"
+    G.EISPH_OneStep(solv, P, B, gs, gb, diagnostics=True)
+    torch.cuda.synchronize()
+    finite = all(bool(torch.isfinite(getattr(P, k)).all()) for k in ("vel", "pres"))
+    passes = {k: v for k, v in timer.ms.items() if k.startswith("eisph_")}
+    log(f"[bench] {steps} steps in {elapsed:.3f}s; per-pass ms: "
+        + ", ".join(f"{k}={v:.2f}" for k, v in passes.items() if v > 0))
+
+    dom = max(passes, key=lambda k: passes[k])
+    count = max(timer.count[dom], 1)
+    avg_s = passes[dom] / count * 1e-3
+    bf, be, bg = EISPH_PASS_BYTES[dom]
+    bytes_launch = bf * P.n + be * L.entries + bg * B.n
+    achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         f"eisph_{case.name}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("kernel") == dom:
+            traffic = int(t["bytes_per_launch"])
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "kernel": dom, "avg_launch_ms": round(avg_s * 1e3, 4),
+                "bytes_per_launch": int(bytes_launch), "launch_times": timed_from}
+    cpu = None if args.no_cpu_baseline else cpu_baseline_cavity(case, args.cpu_steps)
+    line = {
+        "metric": "EISPH cavity point-steps/s", "value": round(P.n * steps / elapsed, 1),
+        "unit": "point-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{case.name}: {case.note}", "fluid": P.n, "ghost": B.n,
+                   "list_entries": L.entries, "dt": solv.dt, "graph_steps": per_graph,
+                   "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "finite": finite,
+        "passes_ms_per_launch": {k: round(v / max(timer.count[k], 1), 4)
+                                 for k, v in passes.items() if v > 0},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline_cavity(case, n_steps: int) -> dict:
+    """The C oracle (OpenMP) on the same cavity: n_steps steps without
+    diagnostics after its fixed grids and KGC are built."""
+    from oracle import eisph as E
+    from oracle import oracle as O
+    from warp_sph_gpt_amd.workloads import cavity_case
+
+    gen, solv, P, B = cavity_case(case.name, device="cpu")
+    oP, oB = E.from_struct(P), E.from_struct(B)
+    gs, gb = E.build_grids(solv, oP, oB, dims=None if case.cells <= 64 else (512, 1, 512))
+    n_steps = max(1, n_steps if case.cells > 64 else 100 * n_steps)
+    t = time.perf_counter()
+    for _ in range(n_steps):
+        E.EISPH_OneStep(solv, oP, oB, gs, gb, diagnostics=False)
+    el = time.perf_counter() - t
+    log(f"[bench] cpu oracle: {n_steps} cavity step(s) in {el:.2f}s on {O.num_threads()} threads")
+    return {"value": round(oP.n * n_steps / el, 1), "unit": "point-steps/s",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": f"{n_steps} step(s) of {case.name} ({oP.n} fluid points), no diagnostics"}
 
 
 def cpu_baseline(wl, n_steps: int, stride: int) -> dict:

@@ -1,6 +1,6 @@
 /*
  * sphdem.h -- C ABI of libsphdem.so, the MI355X (gfx950) implementation of
- * the coupled WCSPH-DEM time step.
+ * the coupled WCSPH-DEM time step and of the Eulerian ISPH cavity step.
  *
  * Reference counterparts are cited as file:line of hojin9908/Warp_SPH_GPT.
  * The reference binds its path through NVIDIA Warp (wp.launch of @wp.kernel
@@ -257,6 +257,13 @@ int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t 
 #define SPHDEM_PASS_STEP_DEM 9
 #define SPHDEM_PASS_OTHER 10
 #define SPHDEM_NUM_PASSES 11
+/* Passes of the Eulerian cavity step (profiled, not tunable). */
+#define SPHDEM_PASS_EISPH_ADVEC 11
+#define SPHDEM_PASS_EISPH_PPE 12
+#define SPHDEM_PASS_EISPH_SOLVE 13
+#define SPHDEM_PASS_EISPH_CORRECT 14
+#define SPHDEM_PASS_EISPH_FINISH 15
+#define SPHDEM_NUM_PROFILED_PASSES 16
 int sphdem_profile_enable(int on);
 int sphdem_profile_read(double *ms, int *counts, int n);
 
@@ -396,34 +403,39 @@ typedef struct {
  * reference re-walks both HashGrids in every pass, KERNEL_EISPH_*.py; the
  * points never move, so the walk is done once).  Row i lists, in the
  * HashGrid visiting order, the fluid and then the ghost neighbours j of
- * fluid point i with 1e-12 < |r_ij|^2 < support^2: id[e] = j for a fluid
- * point, -1 - bj for ghost bj, and geo[e] = (gx, gz, A_ij, V_j), the
+ * fluid point i with 1e-12 < |r_ij|^2 < support^2: id = j for a fluid
+ * point, -1 - bj for ghost bj, and geo = (gx, gz, A_ij, V_j), the
  * KGC-corrected grad_i W_ij (KERNEL_EISPH_KNL.py:52-64), the Laplacian
  * weight 2 V_j (r_ij . g)/r^2 (KERNEL_EISPH_KNL.py:67-71) and V_j = m/rho.
+ * Layout: slices of 64 consecutive rows (one wavefront), entry k of row i
+ * at offset[i / 64] + 64 k + i % 64, so the 64 lanes of a wavefront read
+ * entry k of their rows from consecutive addresses; a slice is as wide as
+ * its longest row (slots past len[i] are never read).
  * The list holds the kgc, m and rho of its build: rebuild it after changing
  * any of them or the positions. */
 typedef struct {
     int n;           /* fluid points of the build */
-    int *offset;     /* [n+1] row starts; offset[n] = entries */
+    int *len;        /* [n+1] entries of each row */
+    int *offset;     /* [slices+1] first slot of each slice; offset[slices] = slots */
     int *id;         /* [cap] */
     void *geo;       /* [cap*4] REAL */
     int cap;
-    int *count;      /* [n+1] build scratch */
-    int *scan_tmp;   /* [sphdem_scan_scratch(n+1)] */
+    int *width;      /* [slices+1] build scratch */
+    int *scan_tmp;   /* [sphdem_scan_scratch(slices+1)] */
     double *part;    /* [sphdem_eisph_part_size(n_fluid, n_ghost)]: the FP64
                         right-hand-side sum, the raw bi[0], per-workgroup sums */
-    int *done;       /* [1] workgroup counter of that sum; zero before the first
-                        call, every call leaves it zero */
 } sphdem_eisph_nbr_t;
+/* Slices of the list of n fluid points: (n + 63) / 64. */
+int sphdem_eisph_slices(int n);
 long sphdem_eisph_part_size(int n_fluid, int n_ghost);
 
 /* KERNEL_EISPH_KNL.py:74-132 Kernel_prepare_kgc: P->kgc from both grids
  * (built from P->pos and B->pos with radius = support). */
 int sphdem_eisph_kgc(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, const sphdem_eisph_t *B,
                      const sphdem_grid_t *gs, const sphdem_grid_t *gb, void *stream);
-/* Neighbour list build, in two calls: _count writes L->offset; the caller
- * reads offset[n] back, sizes id/geo to cap >= offset[n], then _fill writes
- * id/geo (it never writes an entry at or beyond cap). */
+/* Neighbour list build, in two calls: _count writes L->len and L->offset;
+ * the caller reads offset[slices] back, sizes id/geo to cap >= that, then
+ * _fill writes id/geo (it never writes a slot at or beyond cap). */
 int sphdem_eisph_nbr_count(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P,
                            const sphdem_eisph_t *B, const sphdem_grid_t *gs,
                            const sphdem_grid_t *gb, sphdem_eisph_nbr_t *L, void *stream);

@@ -160,12 +160,14 @@ def Kernel_diagnostics_eisph(P, B, gs, gb, support, h):
 
 # ------------------------------------------------------------------ driver
 
-def build_grids(solv, P, B):
-    """EISPH.py:117-122: both fixed grids (grid_slice^3) and the KGC."""
+def build_grids(solv, P, B, dims=None):
+    """EISPH.py:117-122: both fixed grids (grid_slice^3 buckets, or `dims`)
+    and the KGC."""
     dtype = _dt(P)
     g = solv.grid_slice
-    gs = OracleGrid(g, g, g, dtype).build(P.pos, solv.support)
-    gb = OracleGrid(g, g, g, dtype).build(B.pos, solv.support)
+    dims = (g, g, g) if dims is None else dims
+    gs = OracleGrid(*dims, dtype).build(P.pos, solv.support)
+    gb = OracleGrid(*dims, dtype).build(B.pos, solv.support)
     Kernel_prepare_kgc(P, B, gs, gb, solv.support, solv.h)
     return gs, gb
 

@@ -91,9 +91,9 @@ class EisphParams(ctypes.Structure):
 
 
 class EisphNbr(ctypes.Structure):
-    _fields_ = [("n", c_int), ("offset", c_void_p), ("id", c_void_p), ("geo", c_void_p),
-                ("cap", c_int), ("count", c_void_p), ("scan_tmp", c_void_p), ("part", c_void_p),
-                ("done", c_void_p)]
+    _fields_ = [("n", c_int), ("len", c_void_p), ("offset", c_void_p), ("id", c_void_p),
+                ("geo", c_void_p), ("cap", c_int), ("width", c_void_p), ("scan_tmp", c_void_p),
+                ("part", c_void_p)]
 
 
 class TuningStruct(ctypes.Structure):
@@ -138,6 +138,7 @@ _SIGNATURES = {
                              POINTER(GridStruct), POINTER(GridStruct), c_int, c_double, c_void_p,
                              POINTER(WorkStruct), c_void_p], c_int),
     "sphdem_eisph_part_size": ([c_int, c_int], ctypes.c_long),
+    "sphdem_eisph_slices": ([c_int], c_int),
     "sphdem_eisph_kgc": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(GridStruct),
                           POINTER(GridStruct), c_void_p], c_int),
     "sphdem_eisph_nbr_count": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView),
@@ -220,6 +221,8 @@ _SIGNATURES = {
 
 PASS_NAMES = ("grid_build", "fluid_density", "wall_density", "fluid_force", "dem_csr",
               "dem_contact", "porosity", "interaction_dem", "fluid_final", "step_dem", "other")
+PROFILED_PASSES = PASS_NAMES + ("eisph_advec", "eisph_ppe", "eisph_solve", "eisph_correct",
+                                "eisph_finish")
 
 
 class PassTimer:
@@ -231,13 +234,13 @@ class PassTimer:
         return self
 
     def __exit__(self, *exc):
-        n = len(PASS_NAMES)
+        n = len(PROFILED_PASSES)
         ms = (ctypes.c_double * n)()
         cnt = (ctypes.c_int * n)()
         call("sphdem_profile_read", ctypes.addressof(ms), ctypes.addressof(cnt), n)
         call("sphdem_profile_enable", 0)
-        self.ms = {PASS_NAMES[i]: ms[i] for i in range(n)}
-        self.count = {PASS_NAMES[i]: cnt[i] for i in range(n)}
+        self.ms = {PROFILED_PASSES[i]: ms[i] for i in range(n)}
+        self.count = {PROFILED_PASSES[i]: cnt[i] for i in range(n)}
         return False
 
 _lib = None

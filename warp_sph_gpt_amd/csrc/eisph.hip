@@ -7,15 +7,18 @@
 // needs (corrected gradient, Laplacian weight, neighbour volume), in the
 // grid's visiting order.  A step then streams the list: each pass adds the
 // same terms in the same order as the reference's grid walk, so the sums
-// are the reference's sums.
+// are the reference's sums.  The list is sliced by wavefront (64 rows, entry
+// k of the 64 rows in consecutive slots), so every list load of a
+// wavefront is one contiguous 256-byte (id) or 1-KiB (geo) access.
 //
 // Step (sphdem_eisph_step), five launches, one thread per point, ghosts
 // after the fluid points:
 //   advec    u* = u + dt (-(u.grad)u + nu lap u)
-//   ppe      div u*, sum A_ij, sum A_ij p_j^t, b = rho0 div/dt, the FP64
-//            sum of b (last workgroup to finish adds the workgroup sums in a
-//            fixed order); ghosts publish vel_star
-//   solve    b -= mean(b); p = (b + Aijpj)/Aij - gauge
+//   ppe      div u*, sum A_ij, sum A_ij p_j^t, b = rho0 div/dt, per-workgroup
+//            FP64 sums of b; ghosts publish vel_star
+//   solve    the workgroup sums added in a fixed shape (inline for small
+//            clouds, else by a one-workgroup launch before it: the same
+//            bits), b -= mean(b); p = (b + Aijpj)/Aij - gauge
 //   correct  u = u* - dt grad p / rho; ghosts publish pres
 //   finish   ghosts publish vel; fluid diagnostics on request
 // A ghost's velocity 2 vel_bc - u[mirror] and pressure p[mirror] are formed
@@ -39,11 +42,24 @@ template <typename T> EisV<T> eview(const sphdem_eisph_t *a) {
             (T *)a->divergence, (T *)a->residual, a->mirror, a->n};
 }
 template <typename T> struct NbrV {
-    const int *off, *id;
+    const int *len, *off, *id;
     const V4<T> *geo;
 };
 template <typename T> NbrV<T> nview(const sphdem_eisph_nbr_t *L) {
-    return {L->offset, L->id, (const V4<T> *)L->geo};
+    return {L->len, L->offset, L->id, (const V4<T> *)L->geo};
+}
+constexpr int kSlice = 64;
+// Slot of entry k of row i.
+__device__ __forceinline__ int slot0(const int *off, int i) {
+    return off[i / kSlice] + (i % kSlice);
+}
+
+// Visit row i's entries in order: fn(id, geo).
+template <typename T, typename Fn>
+__device__ __forceinline__ void each_entry(const NbrV<T> &L, int i, Fn &&fn) {
+    const int n = L.len[i];
+    int e = slot0(L.off, i);
+    for (int k = 0; k < n; ++k, e += kSlice) fn(L.id[e], L.geo[e]);
 }
 
 // ---------------------------------------------------------------- kernels of the pair
@@ -114,17 +130,27 @@ __global__ void k_eis_kgc(EisV<T> P, EisV<T> B, GridV<T> gs, GridV<T> gb, T supp
     for (int k = 0; k < 4; ++k) P.kgc[4 * i + k] = c[k];
 }
 
-// Entries per fluid point; count[n] = 0 so the exclusive scan of n+1 gives
-// offset[n] = total.
+// Entries per fluid point (len[n] = 0).
 template <typename T>
-__global__ void k_eis_count(EisV<T> P, GridV<T> gs, GridV<T> gb, T support, int *count) {
+__global__ void k_eis_count(EisV<T> P, GridV<T> gs, GridV<T> gb, T support, int *len) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid == 0) count[P.n] = 0;
+    if (tid == 0) len[P.n] = 0;
     if (tid >= P.n) return;
     const int i = gs.ids[tid];
     int c = 0;
     walk_pairs(gs, gb, ld3(P.pos, i), support, [&](int, int, V3<T>, T) { ++c; });
-    count[i] = c;
+    len[i] = c;
+}
+
+// Slots of each slice: 64 x its longest row; width[slices] = 0 so the
+// exclusive scan of slices+1 gives offset[slices] = total slots.
+__global__ void k_eis_width(const int *len, int n, int slices, int *width) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > slices) return;
+    int w = 0;
+    if (s < slices)
+        for (int i = s * kSlice, e = min(n, i + kSlice); i < e; ++i) w = max(w, len[i]);
+    width[s] = w * kSlice;
 }
 
 template <typename T>
@@ -135,7 +161,7 @@ __global__ void k_eis_fill(EisV<T> P, EisV<T> B, GridV<T> gs, GridV<T> gb, T sup
     const int i = gs.ids[tid];
     T c[4];
     for (int k = 0; k < 4; ++k) c[k] = P.kgc[4 * i + k];
-    int e = off[i];
+    int e = slot0(off, i);
     walk_pairs(gs, gb, ld3(P.pos, i), support, [&](int side, int j, V3<T> rij, T r2) {
         const EisV<T> &S = side == 0 ? P : B;
         const V3<T> g = corrected(c, grad_w(rij, r2, h));
@@ -146,7 +172,7 @@ __global__ void k_eis_fill(EisV<T> P, EisV<T> B, GridV<T> gs, GridV<T> gb, T sup
             id[e] = side == 0 ? j : -1 - j;
             geo[e] = V4<T>{g.x, g.z, lap, vol};
         }
-        ++e;
+        e += kSlice;
     });
 }
 
@@ -171,15 +197,13 @@ k_eis_advec(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T dt) {
     if (i >= P.n) return;
     const V3<T> vi = ld3(P.vel, i);
     V3<T> conv = mk3((T)0, (T)0, (T)0), lap = conv;
-    for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
-        const int j = L.id[e];
-        const V4<T> G = L.geo[e];
+    each_entry(L, i, [&](int j, const V4<T> &G) {
         const V3<T> g = gvec(G);
         const V3<T> vj = j >= 0 ? ld3(P.vel, j) : ghost_vec<T, kFly>(B, P.vel, B.vel, -1 - j);
         const V3<T> dv = vj - vi;
         conv = conv + (G.w * dot(vi, g)) * dv;
         lap = lap - G.z * dv;
-    }
+    });
     const V3<T> acc = -conv + nu * lap;
     st3(P.acc, i, acc);
     st3(P.vel_star, i, vi + dt * acc);
@@ -197,24 +221,32 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
     return s;
 }
 
-// KERNEL_EISPH_PPE.py:12-79 Kernel_build_ppe_eisph; the FP64 sum of the raw
-// bi (EISPH.py:72) into part[0] and residual[0], the raw bi[0] into part[1];
-// with kFly the ghost threads publish vel_star (KERNEL_EISPH_BC.py:6-31,
-// predicted).
+// Total of the nb workgroup sums part[2..2+nb): each thread adds one run of
+// consecutive sums, then block_sum.  The shape depends on nb and blockDim
+// only, so every caller gets the same bits.
+__device__ __forceinline__ double sum_partials(const double *part, int nb, double *lds) {
+    const int per = (nb + blockDim.x - 1) / blockDim.x;
+    const int a = threadIdx.x * per, z = min(nb, a + per);
+    double acc = 0.0;
+    for (int k = a; k < z; ++k) acc += part[2 + k];
+    return block_sum(acc, lds);
+}
+
+// KERNEL_EISPH_PPE.py:12-79 Kernel_build_ppe_eisph, with the workgroup sums
+// of the raw bi (for EISPH.py:72) in part[2 + workgroup] and the raw bi[0]
+// in part[1]; with kFly the ghost threads publish vel_star
+// (KERNEL_EISPH_BC.py:6-31, predicted).
 template <typename T, bool kFly>
 __global__ void __launch_bounds__(kEisThreads)
-k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part, int *done) {
+k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part) {
     __shared__ double lds[kEisThreads / 64];
-    __shared__ int last;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double mine = 0.0;
     if (t < P.n) {
         const int i = t;
         const V3<T> vi = ld3(P.vel_star, i);
         T div = 0, A = 0, Ap = 0;
-        for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
-            const int j = L.id[e];
-            const V4<T> G = L.geo[e];
+        each_entry(L, i, [&](int j, const V4<T> &G) {
             const V3<T> g = gvec(G);
             V3<T> vj;
             T pj;
@@ -229,7 +261,7 @@ k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part, int *done
             div = div + G.w * dot(vj - vi, g);
             A = A + G.z;
             Ap = Ap + G.z * pj;
-        }
+        });
         const T bi = rho0 * div / dt;
         P.div[i] = div;
         P.Aij[i] = A;
@@ -242,40 +274,43 @@ k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part, int *done
         st3(B.vel_star, b, (T)2.0 * ld3(B.vel_bc, b) - ld3(P.vel_star, B.mirror[b]));
     }
     const double s = block_sum(mine, lds);
-    if (threadIdx.x == 0) {
-        part[2 + blockIdx.x] = s;
-        __threadfence();
-        last = atomicAdd(done, 1) == (int)gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    // Last workgroup: every other workgroup's sum is visible (released by its
-    // fence before the counter).  Add them in workgroup order.
-    __threadfence();
-    double acc = 0.0;
-    const int nb = gridDim.x;
-    const int per = (nb + blockDim.x - 1) / blockDim.x;
-    const int a = threadIdx.x * per, z = min(nb, a + per);
-    for (int k = a; k < z; ++k)
-        acc += __hip_atomic_load(&part[2 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // consecutive chunks per thread, then the fixed tree of block_sum
-    const double total = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[2 + blockIdx.x] = s;
+}
+
+// EISPH.py:72 the right-hand-side sum: part[0] in FP64, residual[0] rounded
+// to T, as the reference's array_sum leaves it.
+template <typename T>
+__global__ void __launch_bounds__(kEisThreads) k_eis_rhs_sum(EisV<T> P, double *part, int nb) {
+    __shared__ double lds[kEisThreads / 64];
+    const double total = sum_partials(part, nb, lds);
     if (threadIdx.x == 0) {
         part[0] = total;
         P.res[0] = (T)total;
-        *done = 0;
     }
 }
 
 // KERNEL_EISPH_PPE.py:82-125 Kernel_remove_bi_mean and Kernel_solve_ppe_eisph.
-// residual[0] holds the sum rounded to T, as the reference's array_sum
-// leaves it; the gauge row 0 uses its own bi after the mean removal.
-template <typename T>
+// The mean is residual[0] / n with residual[0] the sum rounded to T, as in
+// the reference; the gauge row 0 uses its own bi after the mean removal.
+// kInline: every workgroup adds the nb workgroup sums itself (workgroup 0
+// publishes them); otherwise k_eis_rhs_sum has left the total in part[0].
+template <typename T, bool kInline>
 __global__ void __launch_bounds__(kEisThreads)
-k_eis_solve(EisV<T> P, const double *part) {
+k_eis_solve(EisV<T> P, double *part, int nb) {
+    __shared__ double lds[kEisThreads / 64];
+    double total;
+    if (kInline) {
+        total = sum_partials(part, nb, lds);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            part[0] = total;
+            P.res[0] = (T)total;
+        }
+    } else {
+        total = part[0];
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
-    const T mean = (T)part[0] / (T)P.n;
+    const T mean = (T)total / (T)P.n;
     const T bi = P.bi[i] - mean;
     P.bi[i] = bi;
     T gauge = 0;
@@ -301,14 +336,12 @@ k_eis_correct(EisV<T> P, EisV<T> B, NbrV<T> L, T dt) {
     const int i = t;
     const T pi = P.pres[i];
     V3<T> gp = mk3((T)0, (T)0, (T)0);
-    for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
-        const int j = L.id[e];
-        const V4<T> G = L.geo[e];
+    each_entry(L, i, [&](int j, const V4<T> &G) {
         T pj;
         if (j >= 0) pj = P.pres[j];
         else pj = kFly ? P.pres[B.mirror[-1 - j]] : B.pres[-1 - j];
         gp = gp + (G.w * (pj - pi)) * gvec(G);
-    }
+    });
     const V3<T> old = ld3(P.vel, i);
     const V3<T> v = ld3(P.vel_star, i) - (dt * gp) / P.rho[i];
     st3(P.vel, i, v);
@@ -332,9 +365,7 @@ k_eis_finish(EisV<T> P, EisV<T> B, NbrV<T> L) {
     const V3<T> vi = ld3(P.vel, i);
     const T pi = P.pres[i];
     T div = 0, op = 0;
-    for (int e = L.off[i], end = L.off[i + 1]; e < end; ++e) {
-        const int j = L.id[e];
-        const V4<T> G = L.geo[e];
+    each_entry(L, i, [&](int j, const V4<T> &G) {
         V3<T> vj;
         T pj;
         if (j >= 0) {
@@ -347,7 +378,7 @@ k_eis_finish(EisV<T> P, EisV<T> B, NbrV<T> L) {
         }
         div = div + G.w * dot(vj - vi, gvec(G));
         op = op + G.z * (pi - pj);
-    }
+    });
     P.div[i] = div;
     P.res[i] = op - P.bi[i];
 }
@@ -406,12 +437,13 @@ int check_grids(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P, const s
 int check_list(const sphdem_eisph_t *P, const sphdem_eisph_nbr_t *L) {
     SPHDEM_REQUIRE(L, "neighbour list is NULL");
     SPHDEM_REQUIRE(L->n == P->n, "neighbour list was built for a different number of points");
-    SPHDEM_REQUIRE(L->offset && (L->cap == 0 || (L->id && L->geo)), "neighbour list is not built");
+    SPHDEM_REQUIRE(L->len && L->offset && (L->cap == 0 || (L->id && L->geo)),
+                   "neighbour list is not built");
     return SPHDEM_OK;
 }
 
 int check_sums(const sphdem_eisph_nbr_t *L) {
-    SPHDEM_REQUIRE(L->part && L->done, "neighbour list has no sum buffers");
+    SPHDEM_REQUIRE(L->part, "neighbour list has no sum buffer");
     return SPHDEM_OK;
 }
 
@@ -429,13 +461,18 @@ int check_sums(const sphdem_eisph_nbr_t *L) {
 inline dim3 fluid_grid(int n) { return dim3(blocks_for(n, kEisThreads)); }
 inline dim3 all_grid(int nf, int nb) { return dim3(blocks_for((long)nf + nb, kEisThreads)); }
 
+// Workgroups of a ppe launch over nf fluid (+ nb ghost) threads.
+inline int ppe_groups(int nf, int nb) { return blocks_for((long)nf + nb, kEisThreads); }
+// Below this many workgroup sums the solve kernel adds them itself.
+constexpr int kInlineSums = 64;
+
 template <typename T, bool kFly>
 void launch_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, const sphdem_eisph_t *B,
                 const sphdem_eisph_nbr_t *L, hipStream_t st) {
     const int nb_ghost = kFly ? B->n : 0;
-    hipLaunchKernelGGL((k_eis_ppe<T, kFly>), all_grid(P->n, nb_ghost), dim3(kEisThreads), 0, st,
-                       eview<T>(P), eview<T>(B), nview<T>(L), (T)p->rho0, (T)p->dt, L->part,
-                       L->done);
+    hipLaunchKernelGGL((k_eis_ppe<T, kFly>), dim3(ppe_groups(P->n, nb_ghost)),
+                       dim3(kEisThreads), 0, st, eview<T>(P), eview<T>(B), nview<T>(L),
+                       (T)p->rho0, (T)p->dt, L->part);
 }
 
 }  // namespace
@@ -444,6 +481,8 @@ void launch_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, const sphdem_
 using namespace sphdem;
 
 extern "C" {
+
+int sphdem_eisph_slices(int n) { return n < 0 ? -1 : blocks_for(n, kSlice); }
 
 long sphdem_eisph_part_size(int n_fluid, int n_ghost) {
     if (n_fluid < 0 || n_ghost < 0) return -1;
@@ -469,16 +508,20 @@ int sphdem_eisph_nbr_count(const sphdem_eisph_params_t *p, const sphdem_eisph_t 
     int rc = check_pair(p, P, B);
     if (!rc) rc = check_grids(p, P, B, gs, gb);
     if (rc) return rc;
-    SPHDEM_REQUIRE(L && L->offset && L->count && L->scan_tmp, "neighbour list buffers are NULL");
+    SPHDEM_REQUIRE(L && L->len && L->offset && L->width && L->scan_tmp,
+                   "neighbour list buffers are NULL");
     SPHDEM_REQUIRE(L->n == P->n, "neighbour list was sized for a different number of points");
     hipStream_t st = as_stream(stream);
+    const int slices = blocks_for(P->n, kSlice);
     EIS_DISPATCH(p->precision,
                  hipLaunchKernelGGL(k_eis_count<T>, dim3(blocks_for(P->n + 1, kEisThreads)),
                                     dim3(kEisThreads), 0, st, eview<T>(P), view<T>(gs),
-                                    view<T>(gb), (T)p->support, L->count));
+                                    view<T>(gb), (T)p->support, L->len));
+    hipLaunchKernelGGL(k_eis_width, dim3(blocks_for(slices + 1, kEisThreads)), dim3(kEisThreads),
+                       0, st, L->len, P->n, slices, L->width);
     rc = check_launch("eisph_nbr_count");
     if (rc) return rc;
-    return exclusive_scan(L->count, L->offset, P->n + 1, L->scan_tmp, st);
+    return exclusive_scan(L->width, L->offset, slices + 1, L->scan_tmp, st);
 }
 
 int sphdem_eisph_nbr_fill(const sphdem_eisph_params_t *p, const sphdem_eisph_t *P,
@@ -546,7 +589,12 @@ int sphdem_eisph_build_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
     if (!rc) rc = check_sums(L);
     if (rc) return rc;
     SPHDEM_REQUIRE(P->n > 0, "the pressure equation needs fluid points");
-    EIS_DISPATCH(p->precision, (launch_ppe<T, false>(p, P, B, L, as_stream(stream))));
+    hipStream_t st = as_stream(stream);
+    EIS_DISPATCH(p->precision, ({
+        launch_ppe<T, false>(p, P, B, L, st);
+        hipLaunchKernelGGL(k_eis_rhs_sum<T>, dim3(1), dim3(kEisThreads), 0, st, eview<T>(P),
+                           L->part, ppe_groups(P->n, 0));
+    }));
     return check_launch("eisph_build_ppe");
 }
 
@@ -559,8 +607,8 @@ int sphdem_eisph_solve_ppe(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
     if (rc) return rc;
     SPHDEM_REQUIRE(P->n > 0, "the pressure equation needs fluid points");
     EIS_DISPATCH(p->precision,
-                 hipLaunchKernelGGL(k_eis_solve<T>, fluid_grid(P->n), dim3(kEisThreads), 0,
-                                    as_stream(stream), eview<T>(P), (const double *)L->part));
+                 hipLaunchKernelGGL((k_eis_solve<T, false>), fluid_grid(P->n), dim3(kEisThreads),
+                                    0, as_stream(stream), eview<T>(P), L->part, 0));
     return check_launch("eisph_solve_ppe");
 }
 
@@ -602,11 +650,30 @@ int sphdem_eisph_step(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem_
     EIS_DISPATCH(p->precision, ({
         const EisV<T> Pv = eview<T>(P), Bv = eview<T>(B);
         const NbrV<T> Lv = nview<T>(L);
-        hipLaunchKernelGGL((k_eis_advec<T, true>), gf, blk, 0, st, Pv, Bv, Lv, (T)p->nu,
-                           (T)p->dt);
-        launch_ppe<T, true>(p, P, B, L, st);
-        hipLaunchKernelGGL(k_eis_solve<T>, gf, blk, 0, st, Pv, (const double *)L->part);
-        hipLaunchKernelGGL((k_eis_correct<T, true>), ga, blk, 0, st, Pv, Bv, Lv, (T)p->dt);
+        {
+            ProfScope ps(SPHDEM_PASS_EISPH_ADVEC, st);
+            hipLaunchKernelGGL((k_eis_advec<T, true>), gf, blk, 0, st, Pv, Bv, Lv, (T)p->nu,
+                               (T)p->dt);
+        }
+        {
+            ProfScope ps(SPHDEM_PASS_EISPH_PPE, st);
+            launch_ppe<T, true>(p, P, B, L, st);
+        }
+        {
+            ProfScope ps(SPHDEM_PASS_EISPH_SOLVE, st);
+            const int nb = ppe_groups(P->n, B->n);
+            if (nb <= kInlineSums) {
+                hipLaunchKernelGGL((k_eis_solve<T, true>), gf, blk, 0, st, Pv, L->part, nb);
+            } else {
+                hipLaunchKernelGGL(k_eis_rhs_sum<T>, dim3(1), blk, 0, st, Pv, L->part, nb);
+                hipLaunchKernelGGL((k_eis_solve<T, false>), gf, blk, 0, st, Pv, L->part, nb);
+            }
+        }
+        {
+            ProfScope ps(SPHDEM_PASS_EISPH_CORRECT, st);
+            hipLaunchKernelGGL((k_eis_correct<T, true>), ga, blk, 0, st, Pv, Bv, Lv, (T)p->dt);
+        }
+        ProfScope ps(SPHDEM_PASS_EISPH_FINISH, st);
         if (diagnostics)
             hipLaunchKernelGGL((k_eis_finish<T, true, true>), ga, blk, 0, st, Pv, Bv, Lv);
         else

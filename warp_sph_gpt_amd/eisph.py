@@ -18,6 +18,7 @@ as the reference's kernels do.
 """
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -37,7 +38,10 @@ def _stamp(S) -> tuple:
 
 
 class NeighbourList:
-    """The fixed neighbour list of the fluid points (sphdem_eisph_nbr_t)."""
+    """The fixed neighbour list of the fluid points (sphdem_eisph_nbr_t):
+    slices of 64 rows, entry k of row i in slot offset[i // 64] + 64 k + i % 64."""
+
+    SLICE = 64
 
     def __init__(self, P_sph, P_bnd, grid_sph: HashGrid, grid_bnd: HashGrid, support: float,
                  h: float):
@@ -45,13 +49,14 @@ class NeighbourList:
         n, nb = P_sph.n, P_bnd.n
         lib = _lib.load()
         self.n = n
-        self.offset = torch.zeros(n + 1, dtype=torch.int32, device=dev)
-        self.count = torch.zeros(n + 1, dtype=torch.int32, device=dev)
-        self.scan_tmp = torch.zeros(lib.sphdem_scan_scratch(n + 1), dtype=torch.int32,
+        self.slices = lib.sphdem_eisph_slices(n)
+        self.len = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+        self.offset = torch.zeros(self.slices + 1, dtype=torch.int32, device=dev)
+        self.width = torch.zeros(self.slices + 1, dtype=torch.int32, device=dev)
+        self.scan_tmp = torch.zeros(lib.sphdem_scan_scratch(self.slices + 1), dtype=torch.int32,
                                     device=dev)
         self.part = torch.zeros(lib.sphdem_eisph_part_size(n, nb), dtype=torch.float64,
                                 device=dev)
-        self.done = torch.zeros(1, dtype=torch.int32, device=dev)
         self.id = torch.zeros(0, dtype=torch.int32, device=dev)
         self.geo = torch.zeros((0, 4), dtype=P_sph.pos.dtype, device=dev)
         p = _params(P_sph, support, h)
@@ -59,27 +64,39 @@ class NeighbourList:
         gs, gb = grid_sph.struct(), grid_bnd.struct()
         st = _lib.stream_handle()
         _lib.call("sphdem_eisph_nbr_count", p, Pv, Bv, gs, gb, self.view(), st)
-        entries = int(self.offset[n].item())
-        self.id = torch.empty(entries, dtype=torch.int32, device=dev)
-        self.geo = torch.empty((entries, 4), dtype=P_sph.pos.dtype, device=dev)
+        slots = int(self.offset[self.slices].item())
+        self.id = torch.empty(slots, dtype=torch.int32, device=dev)
+        self.geo = torch.empty((slots, 4), dtype=P_sph.pos.dtype, device=dev)
         _lib.call("sphdem_eisph_nbr_fill", p, Pv, Bv, gs, gb, self.view(), st)
+        self.entries = int(self.len.sum().item())
         self.key = None
 
     @property
-    def entries(self) -> int:
+    def slots(self) -> int:
         return int(self.id.shape[0])
+
+    def rows(self):
+        """Host copy as CSR: (offsets [n+1], ids [entries], geo [entries, 4]),
+        row i's entries in their visiting order."""
+        ln = self.len[:self.n].cpu().numpy().astype("int64")
+        off = self.offset.cpu().numpy().astype("int64")
+        i = np.repeat(np.arange(self.n), ln)
+        k = np.arange(i.size) - np.repeat(np.cumsum(ln) - ln, ln)
+        slot = off[i // self.SLICE] + self.SLICE * k + i % self.SLICE
+        csr = np.concatenate(([0], np.cumsum(ln)))
+        return csr, self.id.cpu().numpy()[slot], self.geo.cpu().numpy()[slot]
 
     def view(self) -> _lib.EisphNbr:
         v = _lib.EisphNbr()
         v.n = self.n
+        v.len = self.len.data_ptr()
         v.offset = self.offset.data_ptr()
-        v.count = self.count.data_ptr()
+        v.width = self.width.data_ptr()
         v.scan_tmp = self.scan_tmp.data_ptr()
         v.part = self.part.data_ptr()
-        v.done = self.done.data_ptr()
         v.id = self.id.data_ptr() if self.id.numel() else None
         v.geo = self.geo.data_ptr() if self.geo.numel() else None
-        v.cap = self.entries
+        v.cap = self.slots
         return v
 
 
@@ -175,13 +192,16 @@ def EISPH_OneStep(solv, P_sph, P_bnd, grid_sph, grid_bnd, diagnostics: bool = Tr
               L.view(), int(bool(diagnostics)), _lib.stream_handle())
 
 
-def prepare(solv, P_sph, P_bnd):
-    """source/EISPH.py:117-126: both fixed grids (grid_slice^3, radius =
-    support), the KGC and the neighbour list; returns (grid_sph, grid_bnd)."""
+def prepare(solv, P_sph, P_bnd, dims=None):
+    """source/EISPH.py:117-126: both fixed grids (grid_slice^3 buckets, or
+    `dims`, radius = support), the KGC and the neighbour list; returns
+    (grid_sph, grid_bnd).  The buckets only order the one-time walk: any
+    dims give the same list."""
     g = solv.grid_slice
+    dims = (g, g, g) if dims is None else tuple(int(d) for d in dims)
     dev = P_sph.pos.device
-    grid_sph = HashGrid(g, g, g, device=dev)
-    grid_bnd = HashGrid(g, g, g, device=dev)
+    grid_sph = HashGrid(*dims, device=dev)
+    grid_bnd = HashGrid(*dims, device=dev)
     grid_sph.build(P_sph.pos, solv.support)
     grid_bnd.build(P_bnd.pos, solv.support)
     Kernel_prepare_kgc(P_sph, P_bnd, grid_sph, grid_bnd, solv.support, solv.h)

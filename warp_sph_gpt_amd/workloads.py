@@ -22,6 +22,13 @@
 All: tank 2 x 0.4 x 1 m, fluid block 0.5 x 0.4 x 0.5 m at the origin (c0-c2),
 3-layer dummy wall, h = 1.3 dx, rho0 = 1000, DEM h = 0.08, K=2e4, eta=4,
 mu=0.3, rho_s=2500, DEM-wall radius 0.01.
+
+The Eulerian lid-driven cavity (Re = 100, 1 x 1 m, lid speed 1 m/s,
+h = 1.3 dx, 3 ghost layers):
+  e0  the reference's shipped cavity: 25 x 25 fluid points, 336 ghosts,
+      dt = 2e-3, 5,000 steps
+  e1  1024 x 1024 fluid points (1,048,576), 12,324 ghosts, dt = 1e-5
+      (the predictor's viscosity limit 0.125 dx^2/nu is 1.19e-5), 500 steps
 """
 from __future__ import annotations
 
@@ -139,3 +146,32 @@ def c3(device="cuda:0", seed=20263, n_inst=64, steps=500, dx=0.01) -> Ensemble:
 
 WORKLOADS = {"c0": c0, "c1": c1, "c2": c2}
 ENSEMBLES = {"c3": c3}
+
+
+@dataclass
+class Cavity:
+    name: str
+    cells: int
+    dt: float
+    steps: int
+    note: str
+
+
+CAVITIES = {
+    "e0": Cavity("e0", 25, 2.0e-3, 5000, "shipped Re=100 cavity, 25x25 fluid points"),
+    "e1": Cavity("e1", 1024, 1.0e-5, 500, "Re=100 cavity, 1024x1024 fluid points"),
+}
+
+
+def cavity_case(name: str, device="cuda:0"):
+    """(generator config, EISPHSolv, fluid, ghosts) of a cavity workload."""
+    from .config import EISPHSolv
+    from .input_gen import CavityPtlGeneration, GenerationConfig
+
+    c = CAVITIES[name]
+    gen = GenerationConfig(device="cpu", dx=1.0 / c.cells, dt=c.dt, n_steps=c.steps,
+                           output_step=c.steps)
+    P, B = CavityPtlGeneration(gen).build()
+    solv = EISPHSolv(device=device, h=gen.h, rho0=gen.rho0, nu=gen.nu, dt=gen.dt,
+                     n_steps=c.steps, output_step=c.steps, gif_save=False)
+    return gen, solv, P.to(device), B.to(device)
