@@ -19,8 +19,9 @@
 //              order; buckets that cannot hold such a term are skipped.
 // Math: EXACT evaluates W with IEEE division and square root as written in
 // kernel/Kernel_KNL.py; FAST uses the hardware square root, fused
-// multiply-adds, a multiply by 1/h and the factored polynomial C u^4 (1+2q)
-// with C applied once per sum.
+// multiply-adds with 1/h folded in, the factored polynomial C u^4 (1+2q)
+// with C applied once per sum, and u = 1 - q/2 clamped to [0, 1] in place
+// of the support test.
 #include "common.hpp"
 
 #include <climits>
@@ -92,18 +93,25 @@ __device__ __forceinline__ int row_slots(const int *lo, const int *hi, int base,
     return 1;
 }
 
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+__device__ __forceinline__ double clamp01(double x) { return fmin(fmax(x, 0.0), 1.0); }
+
+// One term V W(|me - q|, h) of a neighbour sum.  FAST: u = 1 - q/2 clamped
+// to [0, 1] stands in for the support test (W = 0 from q = 2 on), the 1/h
+// scale folds into the two multiply-adds of u and 1 + 2q, and C is applied
+// once per sum; a subject placed far away (1e15) adds nothing.
 template <typename T, bool FAST>
 __device__ __forceinline__ void add_term(T &acc, const V4<T> &me, T qx, T qy, T qz, T V, T thr2,
                                          T h, T inv_h) {
     const T dx = me.x - qx, dy = me.y - qy, dz = me.z - qz;
     if (FAST) {
         const T td = fma(dz, dz, fma(dy, dy, dx * dx));
-        const T qq = fast_sqrt(td) * inv_h;
-        const T u = fma((T)-0.5, qq, (T)1.0);
+        const T r = fast_sqrt(td);
+        const T u = clamp01(fma((T)-0.5 * inv_h, r, (T)1.0));
+        const T pq = fma((T)2.0 * inv_h, r, (T)1.0);
         T t = u * u;
         t = t * t;
-        const T c = fma(V, t * fma((T)2.0, qq, (T)1.0), acc);
-        acc = td < thr2 ? c : acc;
+        acc = fma(V, t * pq, acc);
     } else {
         const T td = dx * dx + dy * dy + dz * dz;
         if (td < thr2) acc = acc + V * wendland_w(msqrt(td), h);
@@ -121,7 +129,8 @@ __device__ __forceinline__ void stream_range(T &acc, const V4<T> &me, int a, int
                                              const V4<T> *pos, const T *Vs, V4<T> *buf, int lane,
                                              T thr2, T h, T inv_h) {
     if (b <= a) return;
-    const T far = (T)1.0e30;
+    // padding: far enough to fail the EXACT test, finite in FAST's terms
+    const T far = FAST ? (T)1.0e15 : (T)1.0e30;
     auto load = [&](int s) {
         V4<T> c = {far, far, far, (T)0};
         if (s < b) {
@@ -196,6 +205,8 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
           Y0 + kOrigin >= 0 && Z0 + kOrigin >= 0))
         return false;
     const T thr = in ? thr2 : (T)0;
+    const T far = (T)1.0e15;
+    const V4<T> mq = (FAST && !in) ? V4<T>{far, far, far, (T)0} : me;
     const int dxy = gs.dx * gs.dy;
     for (int z = Z0; z <= Z1; ++z) {
         const T gz = axis_gap(bzl, bzh, z, w);
@@ -214,8 +225,8 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
             row_slots(gs.lo + gs.off, gs.hi + gs.off, row, gs.dx, xa, xb, a1, b1, a2, b2);
             a1 = uniform_i(a1); b1 = uniform_i(b1);
             a2 = uniform_i(a2); b2 = uniform_i(b2);
-            stream_range<T, FAST>(flt, me, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
-            stream_range<T, FAST>(flt, me, a2, b2, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+            stream_range<T, FAST>(flt, mq, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+            stream_range<T, FAST>(flt, mq, a2, b2, gs.s, Vs, wbuf, lane, thr, h, inv_h);
         }
     }
     return true;
