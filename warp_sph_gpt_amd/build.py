@@ -18,11 +18,14 @@ SOURCES = ("grid.hip", "sph.hip", "dem.hip", "porosity.hip", "mpnn.hip", "eisph.
            "format.hip")
 
 # -ffp-contract=off: every a*b+c rounds twice, as the reference's FP32
-# expressions are written.  Packed FP32 instructions are disabled.
+# expressions are written.  Packed FP32 instructions are disabled.  Matrix
+# instructions take and write ordinary VGPRs (no accumulation-register
+# copies around each product; the porosity tiles read their results at once).
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 

@@ -25,6 +25,7 @@
 #include "common.hpp"
 
 #include <climits>
+#include <type_traits>
 #include <cmath>
 #include <limits>
 
@@ -163,6 +164,138 @@ __device__ __forceinline__ void stream_range(T &acc, const V4<T> &me, int a, int
     }
 }
 
+// FAST stream of sorted slots [a, b): candidates are staged relative to the
+// group centre c as (x', y', z', |q'|^2) with V beside them, so that a term
+// needs |r|^2 = (|m|^2 + |q'|^2) - 2 m.q' (one add, three multiply-adds,
+// with mx2 = -2 m) instead of three differences and a dot product.  With
+// |m|, |q'| <= ~0.2 m the rounding of that form stays ~1e-8 absolute, and
+// |.| under the square root keeps a cancelled value finite.  Lanes outside
+// the group and padding slots sit 1e15 away and add 0.
+template <typename T>
+__device__ __forceinline__ void stream_range_rel(T &acc, const V3<T> &mx2, T m2, int a, int b,
+                                                 const V4<T> *pos, const T *Vs, V4<T> *buf,
+                                                 T *vbuf, int lane, const V3<T> &c, T k1,
+                                                 T k2) {
+    if (b <= a) return;
+    const T far = (T)1.0e15;
+    auto load = [&](int s, T &v) {
+        V4<T> q = {far, far, far, (T)0};
+        v = (T)0;
+        if (s < b) {
+            q = pos[s];
+            v = Vs[s];
+        }
+        q.x -= c.x;
+        q.y -= c.y;
+        q.z -= c.z;
+        q.w = fma(q.z, q.z, fma(q.y, q.y, q.x * q.x));
+        return q;
+    };
+    T vcur;
+    V4<T> cur = load(a + lane, vcur);
+    for (int s0 = a; s0 < b; s0 += 64) {
+        buf[lane] = cur;
+        vbuf[lane] = vcur;
+        __builtin_amdgcn_wave_barrier();
+        T vnxt;
+        const V4<T> nxt = load(s0 + 64 + lane, vnxt);
+        const int cnt = uniform_i(min(64, b - s0));
+        auto term = [&](int j) {
+            const V4<T> q = buf[j];
+            const T td = fma(mx2.x, q.x, fma(mx2.y, q.y, fma(mx2.z, q.z, q.w + m2)));
+            const T r = fast_sqrt(fabs(td));
+            const T u = clamp01(fma(k1, r, (T)1.0));
+            const T pq = fma(k2, r, (T)1.0);
+            T t = u * u;
+            t = t * t;
+            acc = fma(vbuf[j], t * pq, acc);
+        };
+        if (cnt == 64) {
+#pragma unroll 8
+            for (int j = 0; j < 64; ++j) term(j);
+        } else {
+#pragma unroll 4
+            for (int j = 0; j < cnt; ++j) term(j);
+        }
+        __builtin_amdgcn_wave_barrier();
+        cur = nxt;
+        vcur = vnxt;
+    }
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FAST, FP32: the squared distances of a wave's 64 subjects to 16 staged
+// candidates are four 16x16x4 FP32 matrix products (v_mfma_f32_16x16x4f32:
+// lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[4(l/16)+t][l%16]).  Per tile T: A row i = (-2 m_s, 1) of subject
+// s = 16T + i, B column j = (q'_j, |q'_j|^2), C = |m_s|^2, so
+// D = |m|^2 + |q'|^2 - 2 m.q', the form of stream_range_rel.  The kernel
+// term stays on the VALU: each lane adds its 16 (subject, candidate) terms
+// into acc[T][t] (subject 16T + 4(l/16) + t, candidates l%16 mod 16), and the
+// 16 partial sums of a subject are added across lanes once per group.
+__device__ __forceinline__ void stream_range_mfma(float (&acc)[4][4], const float (&a)[4],
+                                                  const f32x4 (&c)[4], int a0, int b0,
+                                                  const V4<float> *pos, const float *Vs,
+                                                  V4<float> *buf, float *vbuf, int lane,
+                                                  const V3<float> &gc, float k1, float k2) {
+    if (b0 <= a0) return;
+    const float far = 1.0e15f;
+    auto load = [&](int s, float &v) {
+        V4<float> q = {far, far, far, 0.0f};
+        v = 0.0f;
+        if (s < b0) {
+            q = pos[s];
+            v = Vs[s];
+        }
+        q.x -= gc.x;
+        q.y -= gc.y;
+        q.z -= gc.z;
+        q.w = fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x));
+        return q;
+    };
+    const int col = lane & 15, comp = lane >> 4;
+    float vcur;
+    V4<float> cur = load(a0 + lane, vcur);
+    for (int s0 = a0; s0 < b0; s0 += 64) {
+        buf[lane] = cur;
+        vbuf[lane] = vcur;
+        __builtin_amdgcn_wave_barrier();
+        float vnxt;
+        const V4<float> nxt = load(s0 + 64 + lane, vnxt);
+        const int nsub = uniform_i((min(64, b0 - s0) + 15) >> 4);
+        for (int sc = 0; sc < nsub; ++sc) {
+            // component comp of candidate 16 sc + col, read at its address (a
+            // four-way select here compiles to a per-lane branch tree)
+            const float bv = reinterpret_cast<const float *>(buf + 16 * sc + col)[comp];
+            const float vj = vbuf[16 * sc + col];
+            auto terms = [&](int T, const f32x4 &d) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float r = fast_sqrt(fabsf(d[t]));
+                    const float u = clamp01(fmaf(k1, r, 1.0f));
+                    const float pq = fmaf(k2, r, 1.0f);
+                    float w = u * u;
+                    w = w * w;
+                    acc[T][t] = fmaf(vj, w * pq, acc[T][t]);
+                }
+            };
+            // all four products in flight before the first result is read;
+            // rows of subjects outside the group sit 1e15 away and add 0
+            // (one code path beats skipping the empty tiles)
+            f32x4 d[4];
+#pragma unroll
+            for (int T = 0; T < 4; ++T)
+                d[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T], bv, c[T], 0, 0, 0);
+#pragma unroll
+            for (int T = 0; T < 4; ++T) terms(T, d[T]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        cur = nxt;
+        vcur = vnxt;
+    }
+}
+
 // 1. near-DEM flag; far subjects get their porosity now.
 template <typename T>
 __global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T> gd, T csup,
@@ -189,7 +322,7 @@ __global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T>
 // holds along some axis.
 template <typename T, bool FAST>
 __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &gs, const T *Vs,
-                             V4<T> *wbuf, int lane, T csup, T thr2, T h, T inv_h) {
+                             V4<T> *wbuf, T *vbuf, int lane, T csup, T thr2, T h, T inv_h) {
     const T big = std::numeric_limits<T>::max();
     const T bxl = uniform_f(wave_min(in ? me.x : big)), bxh = uniform_f(wave_max(in ? me.x : -big));
     const T byl = uniform_f(wave_min(in ? me.y : big)), byh = uniform_f(wave_max(in ? me.y : -big));
@@ -205,9 +338,34 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
           Y0 + kOrigin >= 0 && Z0 + kOrigin >= 0))
         return false;
     const T thr = in ? thr2 : (T)0;
+    // FAST: the subject relative to the group centre (far away if not in)
+    const V3<T> gc = mk3(uniform_f((bxl + bxh) * (T)0.5), uniform_f((byl + byh) * (T)0.5),
+                         uniform_f((bzl + bzh) * (T)0.5));
     const T far = (T)1.0e15;
-    const V4<T> mq = (FAST && !in) ? V4<T>{far, far, far, (T)0} : me;
+    const V3<T> m = in ? mk3(me.x - gc.x, me.y - gc.y, me.z - gc.z) : mk3(far, far, far);
+    const V3<T> mx2 = (T)-2.0 * m;
+    const T m2 = dot(m, m);
+    const T k1 = (T)-0.5 * inv_h, k2 = (T)2.0 * inv_h;
     const int dxy = gs.dx * gs.dy;
+    // FP32 FAST: matrix-core distances (stream_range_mfma); set up the
+    // group's A rows and C values, accumulate, reduce at the end
+    constexpr bool kMfma = FAST && std::is_same<T, float>::value;
+    const int col = lane & 15, comp = lane >> 4;
+    float ma[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 mc[4];
+    float acc[4][4] = {};
+    if constexpr (kMfma) {
+#pragma unroll
+        for (int Tt = 0; Tt < 4; ++Tt) {
+            const int sl = 16 * Tt + col;
+            const float mxs = __shfl((float)m.x, sl, 64), mys = __shfl((float)m.y, sl, 64),
+                        mzs = __shfl((float)m.z, sl, 64);
+            ma[Tt] = comp == 0 ? -2.0f * mxs : comp == 1 ? -2.0f * mys
+                                               : comp == 2 ? -2.0f * mzs : 1.0f;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) mc[Tt][t] = __shfl((float)m2, 16 * Tt + 4 * comp + t, 64);
+        }
+    }
     for (int z = Z0; z <= Z1; ++z) {
         const T gz = axis_gap(bzl, bzh, z, w);
         if (gz * gz > r2c) continue;
@@ -225,9 +383,49 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
             row_slots(gs.lo + gs.off, gs.hi + gs.off, row, gs.dx, xa, xb, a1, b1, a2, b2);
             a1 = uniform_i(a1); b1 = uniform_i(b1);
             a2 = uniform_i(a2); b2 = uniform_i(b2);
-            stream_range<T, FAST>(flt, mq, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
-            stream_range<T, FAST>(flt, mq, a2, b2, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+            if constexpr (kMfma) {
+                const V3<float> gcf = mk3((float)gc.x, (float)gc.y, (float)gc.z);
+                stream_range_mfma(acc, ma, mc, a1, b1, (const V4<float> *)gs.s,
+                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
+                                  (float)k1, (float)k2);
+                stream_range_mfma(acc, ma, mc, a2, b2, (const V4<float> *)gs.s,
+                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
+                                  (float)k1, (float)k2);
+            } else if (FAST) {
+                stream_range_rel(flt, mx2, m2, a1, b1, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
+                stream_range_rel(flt, mx2, m2, a2, b2, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
+            } else {
+                stream_range<T, FAST>(flt, me, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+                stream_range<T, FAST>(flt, me, a2, b2, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+            }
         }
+    }
+    if constexpr (kMfma) {
+        // subject 16T + 4(l/16) + t: add its 16 column partial sums, lanes
+        // l%16 = 0..15 of each quarter (fixed xor tree), then hand the total
+        // to the subject's own lane through the wave's LDS slot
+#pragma unroll
+        for (int Tt = 0; Tt < 4; ++Tt) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                float v = acc[Tt][t];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                v += __shfl_xor(v, 8, 64);
+                acc[Tt][t] = v;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (col == 0) {
+#pragma unroll
+            for (int Tt = 0; Tt < 4; ++Tt)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) ((float *)vbuf)[16 * Tt + 4 * comp + t] = acc[Tt][t];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (in) flt += (T)((const float *)vbuf)[lane];
+        __builtin_amdgcn_wave_barrier();
     }
     return true;
 }
@@ -243,9 +441,11 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
                                                     GridV<T> gd, const T *Vs, const int *flag,
                                                     T csup, T hu, T thr2, int xt, int remap) {
     __shared__ V4<T> stage[256];
+    __shared__ T vstage[256];
     const int k = subject_id(remap);
     const int lane = threadIdx.x & 63;
     V4<T> *wbuf = stage + (threadIdx.x & ~63);
+    T *vbuf = vstage + (threadIdx.x & ~63);
     const bool active = k < gs.n && flag[k] != 0;
     const unsigned long long live = __ballot(active);
     if (live == 0ull) return;
@@ -265,15 +465,16 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
             __ballot(active && ky == ly && kz == lz && ik == li) & todo;
         const bool in = (grp >> lane) & 1ull;
         const GridV<T> gg = at_inst(gs, uniform_i(li));
-        if (!stream_group<T, FAST>(flt, me, in, gg, Vs, wbuf, lane, csup, thr2, h, inv_h)) {
+        if (!stream_group<T, FAST>(flt, me, in, gg, Vs, wbuf, vbuf, lane, csup, thr2, h,
+                                   inv_h)) {
             unsigned long long rest = grp;
             while (rest) {
                 const int l2 = __ffsll((long long)rest) - 1;
                 const int lx = __shfl(kx, l2, 64);
                 const unsigned long long sub = __ballot(kx == lx) & rest;
                 const bool in2 = (sub >> lane) & 1ull;
-                if (!stream_group<T, FAST>(flt, me, in2, gg, Vs, wbuf, lane, csup, thr2, h,
-                                           inv_h) && in2) {
+                if (!stream_group<T, FAST>(flt, me, in2, gg, Vs, wbuf, vbuf, lane, csup, thr2,
+                                           h, inv_h) && in2) {
                     query_runs(at_inst(gs, ik), me.x, me.y, me.z, csup, [&](int lo, int hi) {
                         for (int s = lo; s < hi; ++s) {
                             const V4<T> q = gs.s[s];
