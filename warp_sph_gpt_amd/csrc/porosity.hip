@@ -269,15 +269,17 @@ __device__ __forceinline__ void stream_range_mfma(float (&acc)[4][4], const floa
             // four-way select here compiles to a per-lane branch tree)
             const float bv = reinterpret_cast<const float *>(buf + 16 * sc + col)[comp];
             const float vj = vbuf[16 * sc + col];
+            // V (1 + 2q) = fma(2V/h, r, V): the lane's candidate V folded in once
+            const float kv = k2 * vj;
             auto terms = [&](int T, const f32x4 &d) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const float r = fast_sqrt(fabsf(d[t]));
                     const float u = clamp01(fmaf(k1, r, 1.0f));
-                    const float pq = fmaf(k2, r, 1.0f);
+                    const float pqv = fmaf(kv, r, vj);
                     float w = u * u;
                     w = w * w;
-                    acc[T][t] = fmaf(vj, w * pq, acc[T][t]);
+                    acc[T][t] = fmaf(w, pqv, acc[T][t]);
                 }
             };
             // all four products in flight before the first result is read;
