@@ -263,7 +263,8 @@ int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t 
 #define SPHDEM_PASS_EISPH_SOLVE 13
 #define SPHDEM_PASS_EISPH_CORRECT 14
 #define SPHDEM_PASS_EISPH_FINISH 15
-#define SPHDEM_NUM_PROFILED_PASSES 16
+#define SPHDEM_PASS_EISPH_STEPS 16
+#define SPHDEM_NUM_PROFILED_PASSES 17
 int sphdem_profile_enable(int on);
 int sphdem_profile_read(double *ms, int *counts, int n);
 
@@ -473,6 +474,19 @@ int sphdem_eisph_diagnostics(const sphdem_eisph_params_t *p, sphdem_eisph_t *P,
  * so every field ends the step as the reference's ten launches leave it. */
 int sphdem_eisph_step(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem_eisph_t *B,
                       const sphdem_eisph_nbr_t *L, int diagnostics, void *stream);
+/* source/EISPH.py:131-139, the run_eisph loop between output steps: `steps`
+ * calls of sphdem_eisph_step, diagnostics (if asked) on the last one only,
+ * with the same results field for field.  A cloud for which
+ * sphdem_eisph_one_group returns 1 runs all the steps in one launch of one
+ * workgroup (fields in LDS, barriers between the passes); larger clouds take
+ * the five-launch step `steps` times.  The sum buffer L->part ends as the
+ * last step leaves it. */
+int sphdem_eisph_steps(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem_eisph_t *B,
+                       const sphdem_eisph_nbr_t *L, int steps, int diagnostics, void *stream);
+/* 1 when sphdem_eisph_steps runs n_fluid points and n_ghost ghosts of this
+ * precision (4 or 8) in one workgroup: its LDS share (per point 11 values and
+ * 2 ints, per ghost 3 values and an int) fits in 60 KiB. */
+int sphdem_eisph_one_group(int precision, int n_fluid, int n_ghost);
 
 /* ---- ASCII output (host only) ------------------------------------------ */
 /* The VTK XML frame writer's number formatting (output/output.py:14-24):

@@ -370,3 +370,44 @@ def test_many_workgroup_sums_take_the_separate_reduction():
     for name in FIELDS:
         assert torch.equal(getattr(P, name), getattr(P2, name)), name
         assert rel_err(np_(getattr(P, name)), getattr(oP, name)) <= TOL, name
+
+
+@pytest.mark.parametrize("dtype,dx,fits", [(torch.float32, 0.04, True),
+                                           (torch.float64, 0.1, True),
+                                           (torch.float32, 0.025, False)])
+@pytest.mark.parametrize("steps,diagnostics", [(1, True), (6, False), (6, True)])
+def test_steps_equal_repeated_one_steps(dtype, dx, fits, steps, diagnostics):
+    """EISPH_Steps (one workgroup for the shipped 25 x 25 cavity and a 10 x 10
+    FP64 one; five-launch steps for 40 x 40, beyond one workgroup's LDS) leaves
+    every field, the ghosts and the sum buffer as `steps` EISPH_OneStep calls
+    with diagnostics on the last one only, bit for bit."""
+    gen, solv, P, B, _, _ = cavity(dtype, seed=7, dx=dx, grid_slice=16)
+    rng = np.random.default_rng(11)
+    B.vel_bc = torch.tensor(rng.normal(0, 0.2, (B.n, 3)), dtype=dtype, device=DEV)
+    P2, B2 = P.clone(), B.clone()
+    gs, gb = G.prepare(solv, P, B)
+    gs2, gb2 = G.prepare(solv, P2, B2)
+    assert G.one_group(P, B) == fits
+    for s in range(steps):
+        G.EISPH_OneStep(solv, P, B, gs, gb, diagnostics and s == steps - 1)
+    G.EISPH_Steps(solv, P2, B2, gs2, gb2, steps, diagnostics)
+    torch.cuda.synchronize()
+    for name in FIELDS:
+        assert torch.equal(getattr(P, name), getattr(P2, name)), name
+    for name in ("vel", "vel_star", "pres"):
+        assert torch.equal(getattr(B, name), getattr(B2, name)), f"ghost {name}"
+    L = G.neighbours(P, B, gs, gb, solv.support, solv.h)
+    L2 = G.neighbours(P2, B2, gs2, gb2, solv.support, solv.h)
+    assert torch.equal(L.part, L2.part)
+
+
+def test_steps_zero_is_a_no_op_and_negative_is_refused():
+    gen, solv, P, B, _, _ = cavity(seed=3, dx=0.1, grid_slice=16)
+    gs, gb = G.prepare(solv, P, B)
+    before = [getattr(P, name).clone() for name in FIELDS]
+    G.EISPH_Steps(solv, P, B, gs, gb, 0)
+    torch.cuda.synchronize()
+    for name, t in zip(FIELDS, before):
+        assert torch.equal(getattr(P, name), t), name
+    with pytest.raises(ValueError):
+        G.EISPH_Steps(solv, P, B, gs, gb, -1)

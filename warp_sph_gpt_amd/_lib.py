@@ -162,6 +162,9 @@ _SIGNATURES = {
                                   POINTER(EisphNbr), c_void_p], c_int),
     "sphdem_eisph_step": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(EisphNbr), c_int,
                            c_void_p], c_int),
+    "sphdem_eisph_steps": ([POINTER(EisphParams), POINTER(EisphView), POINTER(EisphView), POINTER(EisphNbr),
+                            c_int, c_int, c_void_p], c_int),
+    "sphdem_eisph_one_group": ([c_int, c_int, c_int], c_int),
     "sphdem_format_g6": ([c_void_p, ctypes.c_long, c_int, c_void_p, ctypes.c_long], ctypes.c_long),
     "sphdem_format_int": ([c_void_p, ctypes.c_long, c_int, c_void_p, ctypes.c_long], ctypes.c_long),
     "sphdem_adam": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
@@ -222,7 +225,7 @@ _SIGNATURES = {
 PASS_NAMES = ("grid_build", "fluid_density", "wall_density", "fluid_force", "dem_csr",
               "dem_contact", "porosity", "interaction_dem", "fluid_final", "step_dem", "other")
 PROFILED_PASSES = PASS_NAMES + ("eisph_advec", "eisph_ppe", "eisph_solve", "eisph_correct",
-                                "eisph_finish")
+                                "eisph_finish", "eisph_steps")
 
 
 class PassTimer:

@@ -177,16 +177,69 @@ __global__ void k_eis_fill(EisV<T> P, EisV<T> B, GridV<T> gs, GridV<T> gb, T sup
 }
 
 // ---------------------------------------------------------------- the step's passes
+// KERNEL_EISPH_BC.py:6-31: a ghost's velocity from its wall velocity and mirror.
+template <typename T> __device__ __forceinline__ V3<T> ghost_of(const V3<T> &vbc, const V3<T> &vm) {
+    return (T)2.0 * vbc - vm;
+}
 // A ghost's field as the pass reads it: stored in B (kFly false, the
 // reference kernels' reads) or formed from its mirror (kFly true).
 template <typename T, bool kFly>
 __device__ __forceinline__ V3<T> ghost_vec(const EisV<T> &B, const T *fluid, const T *stored,
                                            int b) {
-    if (kFly) return (T)2.0 * ld3(B.vel_bc, b) - ld3(fluid, B.mirror[b]);
+    if (kFly) return ghost_of(ld3(B.vel_bc, b), ld3(fluid, B.mirror[b]));
     return ld3(stored, b);
 }
 template <typename T> __device__ __forceinline__ V3<T> gvec(const V4<T> &G) {
     return mk3(G.x, (T)0.0, G.y);
+}
+
+// The per-entry updates and per-point results of the passes.  The pass
+// kernels and the one-workgroup multi-step kernel both use these, so both
+// evaluate the same operations in the same order (-ffp-contract=off).
+// KERNEL_EISPH_force.py:12-72: convection and Laplacian terms.
+template <typename T>
+__device__ __forceinline__ void advec_term(V3<T> &conv, V3<T> &lap, const V3<T> &vi,
+                                           const V3<T> &vj, const V4<T> &G) {
+    const V3<T> dv = vj - vi;
+    conv = conv + (G.w * dot(vi, gvec(G))) * dv;
+    lap = lap - G.z * dv;
+}
+template <typename T>
+__device__ __forceinline__ V3<T> advec_acc(const V3<T> &conv, const V3<T> &lap, T nu) {
+    return -conv + nu * lap;
+}
+// KERNEL_EISPH_PPE.py:12-79 (and the divergence of KERNEL_EISPH_step.py:64-121).
+template <typename T>
+__device__ __forceinline__ void div_term(T &div, const V3<T> &vi, const V3<T> &vj,
+                                         const V4<T> &G) {
+    div = div + G.w * dot(vj - vi, gvec(G));
+}
+template <typename T>
+__device__ __forceinline__ void ppe_term(T &div, T &A, T &Ap, const V3<T> &vi, const V3<T> &vj,
+                                         T pj, const V4<T> &G) {
+    div_term(div, vi, vj, G);
+    A = A + G.z;
+    Ap = Ap + G.z * pj;
+}
+// KERNEL_EISPH_PPE.py:100-125: the gauge from row 0 (its raw b, before the
+// mean removal), then p = (b + sum A_ij p_j) / A_ii - gauge.
+template <typename T> __device__ __forceinline__ T gauge_of(T b0_raw, T mean, T A0, T Ap0) {
+    T gauge = 0;
+    if (mabs(A0) > (T)1.0e-12) gauge = (b0_raw - mean + Ap0) / A0;
+    return gauge;
+}
+template <typename T> __device__ __forceinline__ T solved_p(T b, T p, T A, T Ap, T gauge) {
+    if (mabs(A) > (T)1.0e-12) p = (b + Ap) / A;
+    return p - gauge;
+}
+// KERNEL_EISPH_step.py:12-61: the pressure gradient.
+template <typename T>
+__device__ __forceinline__ void correct_term(V3<T> &gp, T pi, T pj, const V4<T> &G) {
+    gp = gp + (G.w * (pj - pi)) * gvec(G);
+}
+// KERNEL_EISPH_step.py:64-121: the Laplacian of p.
+template <typename T> __device__ __forceinline__ void op_term(T &op, T pi, T pj, const V4<T> &G) {
+    op = op + G.z * (pi - pj);
 }
 
 // KERNEL_EISPH_force.py:12-72 Kernel_advec_vis_eisph
@@ -198,13 +251,10 @@ k_eis_advec(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T dt) {
     const V3<T> vi = ld3(P.vel, i);
     V3<T> conv = mk3((T)0, (T)0, (T)0), lap = conv;
     each_entry(L, i, [&](int j, const V4<T> &G) {
-        const V3<T> g = gvec(G);
         const V3<T> vj = j >= 0 ? ld3(P.vel, j) : ghost_vec<T, kFly>(B, P.vel, B.vel, -1 - j);
-        const V3<T> dv = vj - vi;
-        conv = conv + (G.w * dot(vi, g)) * dv;
-        lap = lap - G.z * dv;
+        advec_term(conv, lap, vi, vj, G);
     });
-    const V3<T> acc = -conv + nu * lap;
+    const V3<T> acc = advec_acc(conv, lap, nu);
     st3(P.acc, i, acc);
     st3(P.vel_star, i, vi + dt * acc);
 }
@@ -247,7 +297,6 @@ k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part) {
         const V3<T> vi = ld3(P.vel_star, i);
         T div = 0, A = 0, Ap = 0;
         each_entry(L, i, [&](int j, const V4<T> &G) {
-            const V3<T> g = gvec(G);
             V3<T> vj;
             T pj;
             if (j >= 0) {
@@ -258,9 +307,7 @@ k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part) {
                 vj = ghost_vec<T, kFly>(B, P.vel_star, B.vel_star, b);
                 pj = P.pres[B.mirror[b]];
             }
-            div = div + G.w * dot(vj - vi, g);
-            A = A + G.z;
-            Ap = Ap + G.z * pj;
+            ppe_term(div, A, Ap, vi, vj, pj, G);
         });
         const T bi = rho0 * div / dt;
         P.div[i] = div;
@@ -271,7 +318,7 @@ k_eis_ppe(EisV<T> P, EisV<T> B, NbrV<T> L, T rho0, T dt, double *part) {
         if (i == 0) part[1] = (double)bi;
     } else if (kFly && t - P.n < B.n) {
         const int b = t - P.n;
-        st3(B.vel_star, b, (T)2.0 * ld3(B.vel_bc, b) - ld3(P.vel_star, B.mirror[b]));
+        st3(B.vel_star, b, ghost_of(ld3(B.vel_bc, b), ld3(P.vel_star, B.mirror[b])));
     }
     const double s = block_sum(mine, lds);
     if (threadIdx.x == 0) part[2 + blockIdx.x] = s;
@@ -313,13 +360,8 @@ k_eis_solve(EisV<T> P, double *part, int nb) {
     const T mean = (T)total / (T)P.n;
     const T bi = P.bi[i] - mean;
     P.bi[i] = bi;
-    T gauge = 0;
-    const T gA = P.Aij[0];
-    if (mabs(gA) > (T)1.0e-12) gauge = ((T)part[1] - mean + P.Aijpj[0]) / gA;
-    T p = P.pres[i];
-    const T A = P.Aij[i];
-    if (mabs(A) > (T)1.0e-12) p = (bi + P.Aijpj[i]) / A;
-    P.pres[i] = p - gauge;
+    const T gauge = gauge_of((T)part[1], mean, P.Aij[0], P.Aijpj[0]);
+    P.pres[i] = solved_p(bi, P.pres[i], P.Aij[i], P.Aijpj[i], gauge);
 }
 
 // KERNEL_EISPH_step.py:12-61 Kernel_correct_eisph; with kFly the ghost
@@ -340,7 +382,7 @@ k_eis_correct(EisV<T> P, EisV<T> B, NbrV<T> L, T dt) {
         T pj;
         if (j >= 0) pj = P.pres[j];
         else pj = kFly ? P.pres[B.mirror[-1 - j]] : B.pres[-1 - j];
-        gp = gp + (G.w * (pj - pi)) * gvec(G);
+        correct_term(gp, pi, pj, G);
     });
     const V3<T> old = ld3(P.vel, i);
     const V3<T> v = ld3(P.vel_star, i) - (dt * gp) / P.rho[i];
@@ -357,7 +399,7 @@ k_eis_finish(EisV<T> P, EisV<T> B, NbrV<T> L) {
     if (t >= P.n) {
         const int b = t - P.n;
         if (kFly && b < B.n)
-            st3(B.vel, b, (T)2.0 * ld3(B.vel_bc, b) - ld3(P.vel, B.mirror[b]));
+            st3(B.vel, b, ghost_of(ld3(B.vel_bc, b), ld3(P.vel, B.mirror[b])));
         return;
     }
     if (!kDiag) return;
@@ -376,8 +418,8 @@ k_eis_finish(EisV<T> P, EisV<T> B, NbrV<T> L) {
             vj = ghost_vec<T, kFly>(B, P.vel, B.vel, b);
             pj = kFly ? P.pres[B.mirror[b]] : B.pres[b];
         }
-        div = div + G.w * dot(vj - vi, gvec(G));
-        op = op + G.z * (pi - pj);
+        div_term(div, vi, vj, G);
+        op_term(op, pi, pj, G);
     });
     P.div[i] = div;
     P.res[i] = op - P.bi[i];
@@ -389,13 +431,232 @@ __global__ void k_eis_dirichlet(EisV<T> P, EisV<T> B, int predicted) {
     if (b >= B.n) return;
     const T *src = predicted ? P.vel_star : P.vel;
     T *dst = predicted ? B.vel_star : B.vel;
-    st3(dst, b, (T)2.0 * ld3(B.vel_bc, b) - ld3(src, B.mirror[b]));
+    st3(dst, b, ghost_of(ld3(B.vel_bc, b), ld3(src, B.mirror[b])));
 }
 
 template <typename T>
 __global__ void k_eis_pressure_boundary(EisV<T> P, EisV<T> B) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b < B.n) B.pres[b] = P.pres[B.mirror[b]];
+}
+
+// ---------------------------------------------------------------- many steps, one workgroup
+// For a small cloud each launch of the five-launch step is a few
+// microseconds of almost no work.  k_eis_steps runs K steps in one
+// workgroup of 1024 threads: the fields a pass reads at other points (vel,
+// vel_star, pres, the ghosts' vel_bc and mirror) and the own-point values
+// carried between passes (rho, bi, Aij, Aijpj, first list slot, row length)
+// live in LDS, only the neighbour list is read from memory, and workgroup
+// barriers separate the passes.  The FP64 sum of b keeps the five-launch
+// shape: the same 64-point wave trees, 256-point group sums, and the
+// group sums added as sum_partials adds them with 256 threads; so every
+// field ends as K calls of the five-launch step leave it.  Ghost fields and
+// the write-only outputs (acc, div, bi, Aij, Aijpj, residual) are stored
+// once, from the last step.
+constexpr int kOneThreads = 1024;
+constexpr int kOneMaxGroups = 64;
+constexpr long kOneMaxLds = 60 * 1024;  // dynamic LDS; the static part is below 1 KiB
+
+template <typename T> long one_group_lds(int nf, int ng) {
+    return (long)nf * (11 * (long)sizeof(T) + 2 * (long)sizeof(int)) +
+           (long)ng * (3 * (long)sizeof(T) + (long)sizeof(int));
+}
+
+// Row entries from LDS-held (first slot, length), list loads of kU entries
+// issued before their terms are added in order.
+template <typename T, typename Fn>
+__device__ __forceinline__ void row_batched(const NbrV<T> &L, int e, int len, Fn &&fn) {
+    constexpr int kU = 4;
+    for (int k = 0; k < len; k += kU, e += kU * kSlice) {
+        int id[kU];
+        V4<T> g[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            id[u] = 0;
+            g[u] = V4<T>{(T)0, (T)0, (T)0, (T)0};
+            if (k + u < len) {
+                id[u] = L.id[e + u * kSlice];
+                g[u] = L.geo[e + u * kSlice];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (k + u < len) fn(id[u], g[u]);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kOneThreads)
+k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, int nb,
+            int steps, int diag) {
+    extern __shared__ double one_lds[];
+    __shared__ double wsum[kOneThreads / 64];
+    __shared__ double gsum[kOneMaxGroups];
+    __shared__ T b0_s;
+    const int n = P.n, ng = B.n, tid = threadIdx.x;
+    constexpr int kWaves = kEisThreads / 64;  // waves of a five-launch workgroup
+    T *vel = reinterpret_cast<T *>(one_lds);
+    T *vst = vel + 3 * n;
+    T *pres = vst + 3 * n;
+    T *rho = pres + n;
+    T *bi = rho + n;
+    T *A = bi + n;
+    T *Ap = A + n;
+    T *vbc = Ap + n;
+    int *mir = reinterpret_cast<int *>(vbc + 3 * ng);
+    int *ent = mir + ng;
+    int *cnt = ent + n;
+    for (int i = tid; i < n; i += kOneThreads) {
+        st3(vel, i, ld3(P.vel, i));
+        st3(vst, i, ld3(P.vel_star, i));
+        pres[i] = P.pres[i];
+        rho[i] = P.rho[i];
+        ent[i] = slot0(L.off, i);
+        cnt[i] = L.len[i];
+    }
+    for (int b = tid; b < ng; b += kOneThreads) {
+        st3(vbc, b, ld3(B.vel_bc, b));
+        mir[b] = B.mirror[b];
+    }
+    __syncthreads();
+    auto ghost = [&](const T *f, int b) { return ghost_of(ld3(vbc, b), ld3(f, mir[b])); };
+
+    double total = 0.0;
+    for (int s = 0; s < steps; ++s) {
+        const bool last = s == steps - 1;
+        // advec (KERNEL_EISPH_force.py:12-72)
+        for (int i = tid; i < n; i += kOneThreads) {
+            const V3<T> vi = ld3(vel, i);
+            V3<T> conv = mk3((T)0, (T)0, (T)0), lap = conv;
+            row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+                const V3<T> vj = j >= 0 ? ld3(vel, j) : ghost(vel, -1 - j);
+                advec_term(conv, lap, vi, vj, G);
+            });
+            st3(vst, i, vi + dt * advec_acc(conv, lap, nu));
+        }
+        __syncthreads();
+        // ppe (KERNEL_EISPH_PPE.py:12-79) over the five-launch thread numbering
+        for (int t0 = 0; t0 < nb * kEisThreads; t0 += kOneThreads) {
+            const int i = t0 + tid;
+            double mine = 0.0;
+            if (i < n) {
+                const V3<T> vi = ld3(vst, i);
+                T div = 0, a = 0, ap = 0;
+                row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+                    V3<T> vj;
+                    T pj;
+                    if (j >= 0) {
+                        vj = ld3(vst, j);
+                        pj = pres[j];
+                    } else {
+                        const int b = -1 - j;
+                        vj = ghost(vst, b);
+                        pj = pres[mir[b]];
+                    }
+                    ppe_term(div, a, ap, vi, vj, pj, G);
+                });
+                const T b = rho0 * div / dt;
+                bi[i] = b;
+                A[i] = a;
+                Ap[i] = ap;
+                if (last && !diag) P.div[i] = div;
+                mine = (double)b;
+                if (i == 0) b0_s = b;
+            }
+            const double w = wave_sum(mine);
+            if ((tid & 63) == 0) wsum[tid >> 6] = w;
+            __syncthreads();
+            if (tid < kOneThreads / kEisThreads) {
+                const int g = t0 / kEisThreads + tid;
+                if (g < nb) {
+                    double acc = 0.0;
+                    for (int k = 0; k < kWaves; ++k) acc += wsum[tid * kWaves + k];
+                    gsum[g] = acc;
+                }
+            }
+            __syncthreads();
+        }
+        // EISPH.py:72, in the shape of sum_partials over 256 threads
+        if (tid < kEisThreads) {
+            const int per = (nb + kEisThreads - 1) / kEisThreads;
+            const int a = tid * per, z = min(nb, a + per);
+            double acc = 0.0;
+            for (int k = a; k < z; ++k) acc += gsum[k];
+            const double w = wave_sum(acc);
+            if ((tid & 63) == 0) wsum[tid >> 6] = w;
+        }
+        __syncthreads();
+        total = 0.0;
+        for (int k = 0; k < kWaves; ++k) total += wsum[k];
+        // solve (KERNEL_EISPH_PPE.py:82-125)
+        const T mean = (T)total / (T)n;
+        const T gauge = gauge_of(b0_s, mean, A[0], Ap[0]);
+        for (int i = tid; i < n; i += kOneThreads) {
+            const T b = bi[i] - mean;
+            bi[i] = b;
+            pres[i] = solved_p(b, pres[i], A[i], Ap[i], gauge);
+            if (last) {
+                P.bi[i] = b;
+                P.Aij[i] = A[i];
+                P.Aijpj[i] = Ap[i];
+            }
+        }
+        if (last && tid == 0) {
+            P.res[0] = (T)total;
+            part[0] = total;
+            part[1] = (double)b0_s;
+        }
+        __syncthreads();
+        // correct (KERNEL_EISPH_step.py:12-61)
+        for (int i = tid; i < n; i += kOneThreads) {
+            const T pi = pres[i];
+            V3<T> gp = mk3((T)0, (T)0, (T)0);
+            row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+                correct_term(gp, pi, j >= 0 ? pres[j] : pres[mir[-1 - j]], G);
+            });
+            const V3<T> old = ld3(vel, i);
+            const V3<T> v = ld3(vst, i) - (dt * gp) / rho[i];
+            st3(vel, i, v);
+            if (last) st3(P.acc, i, (v - old) / dt);
+        }
+        __syncthreads();
+    }
+    // diagnostics of the last step (KERNEL_EISPH_step.py:64-121)
+    if (diag) {
+        for (int i = tid; i < n; i += kOneThreads) {
+            const V3<T> vi = ld3(vel, i);
+            const T pi = pres[i];
+            T div = 0, op = 0;
+            row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+                V3<T> vj;
+                T pj;
+                if (j >= 0) {
+                    vj = ld3(vel, j);
+                    pj = pres[j];
+                } else {
+                    const int b = -1 - j;
+                    vj = ghost(vel, b);
+                    pj = pres[mir[b]];
+                }
+                div_term(div, vi, vj, G);
+                op_term(op, pi, pj, G);
+            });
+            P.div[i] = div;
+            P.res[i] = op - bi[i];
+        }
+    }
+    for (int i = tid; i < n; i += kOneThreads) {
+        st3(P.vel, i, ld3(vel, i));
+        st3(P.vel_star, i, ld3(vst, i));
+        P.pres[i] = pres[i];
+    }
+    // KERNEL_EISPH_BC.py:6-47 from the final fields
+    for (int b = tid; b < ng; b += kOneThreads) {
+        st3(B.vel_star, b, ghost(vst, b));
+        B.pres[b] = pres[mir[b]];
+        st3(B.vel, b, ghost(vel, b));
+    }
+    if (tid < nb) part[2 + tid] = gsum[tid];
 }
 
 // ---------------------------------------------------------------- checks
@@ -680,6 +941,40 @@ int sphdem_eisph_step(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem_
             hipLaunchKernelGGL((k_eis_finish<T, true, false>), ga, blk, 0, st, Pv, Bv, Lv);
     }));
     return check_launch("eisph_step");
+}
+
+int sphdem_eisph_one_group(int precision, int n_fluid, int n_ghost) {
+    if (n_fluid <= 0 || n_ghost < 0) return 0;
+    const long bytes = precision == 8 ? one_group_lds<double>(n_fluid, n_ghost)
+                                      : one_group_lds<float>(n_fluid, n_ghost);
+    return bytes <= kOneMaxLds && ppe_groups(n_fluid, n_ghost) <= kOneMaxGroups;
+}
+
+int sphdem_eisph_steps(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem_eisph_t *B,
+                       const sphdem_eisph_nbr_t *L, int steps, int diagnostics, void *stream) {
+    int rc = check_pair(p, P, B);
+    if (!rc) rc = check_list(P, L);
+    if (!rc) rc = check_sums(L);
+    if (rc) return rc;
+    SPHDEM_REQUIRE(steps >= 0, "negative step count");
+    SPHDEM_REQUIRE(P->n > 0, "the pressure equation needs fluid points");
+    if (steps == 0) return SPHDEM_OK;
+    if (!sphdem_eisph_one_group(p->precision, P->n, B->n)) {
+        for (int s = 0; s < steps; ++s) {
+            rc = sphdem_eisph_step(p, P, B, L, diagnostics && s == steps - 1, stream);
+            if (rc) return rc;
+        }
+        return SPHDEM_OK;
+    }
+    hipStream_t st = as_stream(stream);
+    const int nb = ppe_groups(P->n, B->n);
+    ProfScope ps(SPHDEM_PASS_EISPH_STEPS, st);
+    EIS_DISPATCH(p->precision,
+                 hipLaunchKernelGGL(k_eis_steps<T>, dim3(1), dim3(kOneThreads),
+                                    (size_t)one_group_lds<T>(P->n, B->n), st, eview<T>(P),
+                                    eview<T>(B), nview<T>(L), (T)p->nu, (T)p->rho0, (T)p->dt,
+                                    L->part, nb, steps, diagnostics ? 1 : 0));
+    return check_launch("eisph_steps");
 }
 
 }  // extern "C"

@@ -14,7 +14,9 @@ rebuilt, the KGC is recomputed, or m or rho change.
 
 `EISPH_OneStep` is five launches (sphdem_eisph_step); the kernel-level
 functions run one reference launch each and read the ghost fields as stored,
-as the reference's kernels do.
+as the reference's kernels do.  `EISPH_Steps` runs the steps between two
+output steps (sphdem_eisph_steps): one launch of one workgroup for a small
+cloud, with the results of as many `EISPH_OneStep` calls.
 """
 from __future__ import annotations
 
@@ -192,6 +194,29 @@ def EISPH_OneStep(solv, P_sph, P_bnd, grid_sph, grid_bnd, diagnostics: bool = Tr
               L.view(), int(bool(diagnostics)), _lib.stream_handle())
 
 
+def EISPH_Steps(solv, P_sph, P_bnd, grid_sph, grid_bnd, steps: int,
+                diagnostics: bool = True) -> None:
+    """`steps` calls of EISPH_OneStep, diagnostics (if asked) on the last
+    only: the run_eisph loop between output steps (source/EISPH.py:131-139).
+    Clouds that fit one workgroup's LDS (`one_group`) run in one launch."""
+    steps = int(steps)
+    if steps < 0:
+        raise ValueError("steps must be >= 0")
+    L = neighbours(P_sph, P_bnd, grid_sph, grid_bnd, solv.support, solv.h)
+    p = _lib.eisph_params(_lib.precision_of(P_sph.pos.dtype), solv.h, solv.rho0, solv.nu,
+                          solv.dt)
+    p.support = float(solv.support)
+    _lib.call("sphdem_eisph_steps", p, _lib.eisph_view(P_sph), _lib.eisph_view(P_bnd, "P_bnd"),
+              L.view(), steps, int(bool(diagnostics)), _lib.stream_handle())
+
+
+def one_group(P_sph, P_bnd) -> bool:
+    """True when EISPH_Steps runs this cloud in one workgroup."""
+    return bool(_lib.load().sphdem_eisph_one_group(_lib.precision_of(P_sph.pos.dtype),
+                                                  int(P_sph.pos.shape[0]),
+                                                  int(P_bnd.pos.shape[0])))
+
+
 def prepare(solv, P_sph, P_bnd, dims=None):
     """source/EISPH.py:117-126: both fixed grids (grid_slice^3 buckets, or
     `dims`, radius = support), the KGC and the neighbour list; returns
@@ -218,12 +243,14 @@ def run_eisph(solv, P_sph, P_bnd, progress=None):
     grid_sph, grid_bnd = prepare(solv, P_sph, P_bnd)
     positions = P_sph.pos.detach().cpu().numpy().copy()
     frames, times = [P_sph.vel.detach().cpu().numpy().copy()], [0.0]
-    for step in range(1, solv.n_steps + 1):
-        store = (solv.output_step > 0 and step % solv.output_step == 0) or step == solv.n_steps
-        EISPH_OneStep(solv, P_sph, P_bnd, grid_sph, grid_bnd, diagnostics=store)
-        if store:
-            frames.append(P_sph.vel.detach().cpu().numpy().copy())
-            times.append(step * solv.dt)
-            if progress is not None:
-                progress(step, solv.n_steps)
+    step, n, every = 0, int(solv.n_steps), int(solv.output_step)
+    while step < n:
+        # the next step the reference stores at (a multiple of output_step, or the last)
+        store = min(n, (step // every + 1) * every) if every > 0 else n
+        EISPH_Steps(solv, P_sph, P_bnd, grid_sph, grid_bnd, store - step, diagnostics=True)
+        step = store
+        frames.append(P_sph.vel.detach().cpu().numpy().copy())
+        times.append(step * solv.dt)
+        if progress is not None:
+            progress(step, n)
     return positions, frames, times
