@@ -125,7 +125,13 @@ typedef struct {
     int *fl_list;                   /* [2*nf] porosity targets, flag offsets */
     int *fl_count;                  /* [1] number of porosity targets */
     int *fl_scan;                   /* [sphdem_scan_scratch(nf)] */
-    int cap_f, cap_b;
+    /* [cap_d] or NULL: per-DEM flags, all zero between steps.  In the fused
+     * coupled step the porosity mark sets dem_near[b] = 1 where some fluid
+     * point lies within 2 h_b, and the DEM interpolation skips the fluid
+     * search of the other DEMs (it would find no term) and clears the flags
+     * it read.  NULL (or cap_d below the DEM count): no skipping. */
+    int *dem_near;
+    int cap_f, cap_b, cap_d;
 } sphdem_work_t;
 
 /* Library identity and diagnostics. */

@@ -142,3 +142,36 @@ def test_shadow_reports_every_field():
     assert rep["totals_rel"]["fluid_ke"] < 5e-3
     assert rep["totals_rel"]["front"] < 1e-5
     assert rep["fluid"]["pos"] < 1e-5
+
+
+@pytest.mark.parametrize("fast_math", [False, True], ids=["exact", "fast"])
+def test_dem_near_flags_skip_only_dry_dems(fast_math):
+    """The porosity mark's per-DEM flags let the DEM interpolation skip DEMs
+    with no fluid within 2 h.  DEMs inside the fluid, DEMs far above it (whose
+    search boxes fold onto fluid buckets) and one exactly on the 2 h edge:
+    five coupled steps with the flags and without them leave every fluid and
+    DEM field equal bit for bit, and the flags all zero after each step."""
+    gen, structs = wet_structs(seed=6)
+    P, B, D, W = structs
+    dpos = np_(D.pos).astype(np.float64)
+    dpos[::2, 2] += 1.1   # every other sphere well above the free surface
+    top = float(np_(P.pos)[:, 2].max())
+    dpos[1] = [0.2, 0.2, top + 2.0 * float(np_(D.h)[1])]
+    D.pos = torch.tensor(dpos, dtype=torch.float32)
+    solv = solv_for(gen)
+    flagged = CoupledEngine(solv, *(s.to(DEV) for s in structs), fast_math=fast_math)
+    plain = CoupledEngine(solv, *(s.to(DEV) for s in structs), fast_math=fast_math)
+    plain.ws.cap_d = 0   # the library then runs every DEM's search
+    assert flagged.ws.cap_d >= D.n
+    for _ in range(5):
+        flagged.step()
+        plain.step()
+        torch.cuda.synchronize()
+        assert int(flagged.ws.dem_near.abs().sum()) == 0
+    for name in ("pos", "vel", "acc", "rho", "pres", "porosity"):
+        assert torch.equal(getattr(flagged.P, name), getattr(plain.P, name)), name
+    for name in ("pos", "vel", "omega", "force", "torque", "flt_s", "porosity", "drag",
+                 "pressure_force"):
+        assert torch.equal(getattr(flagged.D, name), getattr(plain.D, name)), name
+    dry = np_(flagged.D.flt_s) == 0
+    assert dry.any() and not dry.all()

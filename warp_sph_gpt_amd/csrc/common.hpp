@@ -243,34 +243,48 @@ __device__ __forceinline__ int fold(int c, int dim) {
 // (y, z) row are adjacent in sorted order, so a row is one range, or two
 // when it wraps around the folded grid.  fn(lo, hi) sees the points in the
 // same order as Warp's per-bucket iteration.
-template <typename T, typename Fn>
-__device__ __forceinline__ void query_runs(const GridV<T> &g, T px, T py, T pz, T r, Fn &&fn) {
+// The bucket box of a query: virtual cells [x0, x1] x [y0, y1] x [z0, z1],
+// at most the grid's dims along each axis.
+struct QueryBox {
+    int x0, x1, y0, y1, z0, z1;
+};
+template <typename T>
+__device__ __forceinline__ QueryBox query_box(const GridV<T> &g, T px, T py, T pz, T r) {
     const T inv = g.inv;
-    const int x0 = (int)((px - r) * inv), y0 = (int)((py - r) * inv), z0 = (int)((pz - r) * inv);
-    const int x1 = min((int)((px + r) * inv), x0 + g.dx - 1);
-    const int y1 = min((int)((py + r) * inv), y0 + g.dy - 1);
-    const int z1 = min((int)((pz + r) * inv), z0 + g.dz - 1);
-    if (g.n == 0) return;
-    const int dxy = g.dx * g.dy;
+    QueryBox b;
+    b.x0 = (int)((px - r) * inv);
+    b.y0 = (int)((py - r) * inv);
+    b.z0 = (int)((pz - r) * inv);
+    b.x1 = min((int)((px + r) * inv), b.x0 + g.dx - 1);
+    b.y1 = min((int)((py + r) * inv), b.y0 + g.dy - 1);
+    b.z1 = min((int)((pz + r) * inv), b.z0 + g.dz - 1);
+    return b;
+}
+// The slot ranges of one (y, z) row of a query box, in Warp's order.
+template <typename T, typename Fn>
+__device__ __forceinline__ void row_runs(const GridV<T> &g, const QueryBox &b, int y, int z,
+                                         Fn &&fn) {
     const int *lo = g.lo + g.off, *hi = g.hi + g.off;
-    const int len = x1 - x0 + 1;
-    for (int z = z0; z <= z1; ++z) {
-        const int cz = fold(z, g.dz);
-        for (int y = y0; y <= y1; ++y) {
-            const int base = cz * dxy + fold(y, g.dy) * g.dx;
-            if (x0 + kOrigin >= 0) {
-                const int cx = (x0 + kOrigin) % g.dx;
-                const int first = min(len, g.dx - cx);
-                if (first > 0) fn(lo[base + cx], hi[base + cx + first - 1]);
-                if (len > first) fn(lo[base], hi[base + len - first - 1]);
-            } else {
-                for (int x = x0; x <= x1; ++x) {
-                    const int c = base + fold(x, g.dx);
-                    fn(lo[c], hi[c]);
-                }
-            }
+    const int len = b.x1 - b.x0 + 1;
+    const int base = fold(z, g.dz) * (g.dx * g.dy) + fold(y, g.dy) * g.dx;
+    if (b.x0 + kOrigin >= 0) {
+        const int cx = (b.x0 + kOrigin) % g.dx;
+        const int first = min(len, g.dx - cx);
+        if (first > 0) fn(lo[base + cx], hi[base + cx + first - 1]);
+        if (len > first) fn(lo[base], hi[base + len - first - 1]);
+    } else {
+        for (int x = b.x0; x <= b.x1; ++x) {
+            const int c = base + fold(x, g.dx);
+            fn(lo[c], hi[c]);
         }
     }
+}
+template <typename T, typename Fn>
+__device__ __forceinline__ void query_runs(const GridV<T> &g, T px, T py, T pz, T r, Fn &&fn) {
+    const QueryBox b = query_box(g, px, py, pz, r);
+    if (g.n == 0) return;
+    for (int z = b.z0; z <= b.z1; ++z)
+        for (int y = b.y0; y <= b.y1; ++y) row_runs(g, b, y, z, fn);
 }
 
 // Visit slots a, a+step, ... < b in order, loading the records of U slots

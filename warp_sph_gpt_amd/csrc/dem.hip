@@ -9,6 +9,10 @@
 namespace sphdem {
 
 constexpr int kThreads = 256;
+// Threads of the one-workgroup-per-DEM fluid search: few DEMs have fluid
+// near, and each of those walks ~2e5 candidates, so the search of one DEM
+// wants many waves in flight.
+constexpr int kDemThreads = 1024;
 
 // Declared in sph.hip.
 template <typename T>
@@ -31,7 +35,8 @@ int check_work(const sphdem_work_t *w, int nf, int nb);
 template <typename T>
 int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
                           const sphdem_grid_t *gs, const sphdem_grid_t *gd,
-                          const sphdem_work_t *w, double csup, double hu, hipStream_t st);
+                          const sphdem_work_t *w, double csup, double hu, int *dem_near,
+                          hipStream_t st);
 
 // ---------------------------------------------------------------- contact law
 // kernel/KERNEL_DEM_force.py:25-74 DEM_contact: linear spring-dashpot with
@@ -266,10 +271,16 @@ __global__ __launch_bounds__(kThreads) void k_porosity(Prm<T> p, SphV<T> P, DemV
 
 // kernel/KERNEL_SPHDEM_interaction.py:118-187, one workgroup per DEM
 // subject: the fluid sums are split over the workgroup's threads and added
-// in a fixed order (wave shuffles, then the four waves in turn).
+// in a fixed order (wave shuffles, then the waves in turn).
+// With `near` (the porosity mark's per-DEM flags of this step): a DEM whose
+// flag is 0 has no fluid point within 2 h, since the mark tested exactly
+// this distance between the same stored positions (the DEM's grid copy is
+// checked to equal D.pos), so its search would add no term; it gets the
+// no-fluid outputs at once.  The flags read are cleared for the next step.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_interaction_dem(Prm<T> p, DemV<T> D, GridV<T> gs,
-                                                              GridV<T> gd, WorkV<T> w) {
+__global__ __launch_bounds__(kDemThreads) void k_interaction_dem(Prm<T> p, DemV<T> D, GridV<T> gs,
+                                                              GridV<T> gd, WorkV<T> w,
+                                                              int *near) {
     const int t = blockIdx.x;
     if (t >= gd.n) return;
     const int a = gd.ids[t];
@@ -278,28 +289,47 @@ __global__ __launch_bounds__(kThreads) void k_interaction_dem(Prm<T> p, DemV<T> 
     gd = at_inst(gd, ia);
     const V3<T> ra = ld3(D.pos, a);
     const T h = D.h[a], sup = (T)2.0 * h;
-    T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // flt, rho_f, vel_f xyz, pgf xyz
-    query_runs(gs, ra.x, ra.y, ra.z, sup, [&](int lo, int hi) {
-        for (int s = lo + (int)threadIdx.x; s < hi; s += blockDim.x) {
-            const V4<T> q = gs.s[s];
-            const T dist = length(mk3(ra.x - q.x, ra.y - q.y, ra.z - q.z));
-            if (dist < sup) {
-                const T vw = w.fl_V[s] * wendland_w(dist, h);
-                const V4<T> pr = w.fl_pgf[s];
-                const V4<T> v = w.fl_vel[s];
-                acc[0] = acc[0] + vw;
-                acc[1] = acc[1] + pr.w * vw;
-                acc[2] = acc[2] + v.x * vw;
-                acc[3] = acc[3] + v.y * vw;
-                acc[4] = acc[4] + v.z * vw;
-                acc[5] = acc[5] + pr.x * vw;
-                acc[6] = acc[6] + pr.y * vw;
-                acc[7] = acc[7] + pr.z * vw;
+    if (near) {
+        const V4<T> own = gd.s[t];
+        if (near[a] == 0 && own.x == ra.x && own.y == ra.y && own.z == ra.z) {
+            if (threadIdx.x == 0) {
+                D.flt_s[a] = (T)0;
+                st3(D.drag, a, mk3<T>(0, 0, 0));
+                st3(D.pressure_force, a, mk3<T>(0, 0, 0));
+                D.porosity[a] = p.eps_max;
             }
+            return;
         }
-    });
-    __shared__ T red[kThreads / 64][8];
+    }
+    T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // flt, rho_f, vel_f xyz, pgf xyz
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // Each wave takes whole rows of the query box (wave w: rows w, w + 16,
+    // ...), its lanes striding each row's slots, so the rows' bucket-bound
+    // loads overlap instead of forming one chain.
+    const QueryBox qb = query_box(gs, ra.x, ra.y, ra.z, sup);
+    const int ny = qb.y1 - qb.y0 + 1, rows = gs.n ? (qb.z1 - qb.z0 + 1) * ny : 0;
+    for (int r = wid; r < rows; r += kDemThreads / 64) {
+        row_runs(gs, qb, qb.y0 + r % ny, qb.z0 + r / ny, [&](int lo, int hi) {
+            for (int s = lo + lane; s < hi; s += 64) {
+                const V4<T> q = gs.s[s];
+                const T dist = length(mk3(ra.x - q.x, ra.y - q.y, ra.z - q.z));
+                if (dist < sup) {
+                    const T vw = w.fl_V[s] * wendland_w(dist, h);
+                    const V4<T> pr = w.fl_pgf[s];
+                    const V4<T> v = w.fl_vel[s];
+                    acc[0] = acc[0] + vw;
+                    acc[1] = acc[1] + pr.w * vw;
+                    acc[2] = acc[2] + v.x * vw;
+                    acc[3] = acc[3] + v.y * vw;
+                    acc[4] = acc[4] + v.z * vw;
+                    acc[5] = acc[5] + pr.x * vw;
+                    acc[6] = acc[6] + pr.y * vw;
+                    acc[7] = acc[7] + pr.z * vw;
+                }
+            }
+        });
+    }
+    __shared__ T red[kDemThreads / 64][8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const T v = wave_sum(acc[c]);
@@ -311,7 +341,7 @@ __global__ __launch_bounds__(kThreads) void k_interaction_dem(Prm<T> p, DemV<T> 
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         T s = red[0][c];
-        for (int q = 1; q < kThreads / 64; ++q) s = s + red[q][c];
+        for (int q = 1; q < kDemThreads / 64; ++q) s = s + red[q][c];
         tot[c] = s;
     }
     const T flt = tot[0], rho_f = tot[1];
@@ -324,6 +354,7 @@ __global__ __launch_bounds__(kThreads) void k_interaction_dem(Prm<T> p, DemV<T> 
             if (dist < sup) solid = solid + D.volume[gd.ids[s]] * wendland_w(dist, h);
         }
     });
+    if (near) near[a] = 0;
     D.flt_s[a] = flt;
     st3(D.drag, a, mk3<T>(0, 0, 0));
     st3(D.pressure_force, a, mk3<T>(0, 0, 0));
@@ -512,24 +543,37 @@ static void launch_porosity(const sphdem_params_t *p, const sphdem_sph_t *P,
 template <typename T>
 static int porosity(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
                     const sphdem_grid_t *gs, const sphdem_grid_t *gd, const sphdem_work_t *w,
-                    double csup, double hu, hipStream_t st) {
+                    double csup, double hu, hipStream_t st, int *dem_near = nullptr) {
     if (hu > 0.0) {
         SPHDEM_REQUIRE(w->fl_flag && w->fl_list && w->fl_count && w->fl_scan,
                        "porosity work buffers are NULL");
-        return launch_porosity_tiled<T>(p, P, D, gs, gd, w, csup, hu, st);
+        return launch_porosity_tiled<T>(p, P, D, gs, gd, w, csup, hu, dem_near, st);
     }
     launch_porosity<T>(p, P, D, gs, gd, w, csup, hu, st);
     return check_launch("porosity");
 }
 
+// The per-DEM flags of the tiled porosity mark for this step, or null: they
+// need the work array and a mark query (radius csup) that spans fewer DEM
+// grid buckets than the grid holds along every axis, so that it visits every
+// DEM within csup.
+static int *dem_near_flags(const sphdem_params_t *p, const sphdem_dem_t *D,
+                           const sphdem_grid_t *gd, const sphdem_work_t *w, double csup) {
+    if (!(p->h_uniform > 0.0) || !w->dem_near || w->cap_d < D->n || D->n == 0) return nullptr;
+    const double span = 2.0 * csup / gd->cell_width + 3.0;
+    if (span > gd->dim_x || span > gd->dim_y || span > gd->dim_z) return nullptr;
+    return w->dem_near;
+}
+
 template <typename T>
 static void launch_interaction_dem(const sphdem_params_t *p, const sphdem_dem_t *D,
                                    const sphdem_grid_t *gs, const sphdem_grid_t *gd,
-                                   const sphdem_work_t *w, hipStream_t st) {
+                                   const sphdem_work_t *w, hipStream_t st,
+                                   int *near = nullptr) {
     if (gd->n == 0) return;
     ProfScope prof(SPHDEM_PASS_INTERACTION_DEM, st);
-    hipLaunchKernelGGL(k_interaction_dem<T>, dim3(gd->n), dim3(kThreads), 0, st, prm<T>(p),
-                       view<T>(D), view<T>(gs), view<T>(gd), view<T>(w));
+    hipLaunchKernelGGL(k_interaction_dem<T>, dim3(gd->n), dim3(kDemThreads), 0, st, prm<T>(p),
+                       view<T>(D), view<T>(gs), view<T>(gd), view<T>(w), near);
 }
 
 template <typename T, bool INTER, bool STEP>
@@ -589,9 +633,10 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
                                 p->boundary_radius_max, 0.0, p->dt, w->flags, st);
         if (all) launch_copy_csr<T>(D, true, st);
         const double csup = 2.0 * p->h_max;
-        rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st);
+        int *near = dem_near_flags(p, D, gd, w, csup);
+        rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near);
         if (rc) return rc;
-        launch_interaction_dem<T>(p, D, gs, gd, w, st);
+        launch_interaction_dem<T>(p, D, gs, gd, w, st, near);
         launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st);
         launch_step_dem<T>(p, D, st);
     }

@@ -301,7 +301,7 @@ __device__ __forceinline__ void stream_range_mfma(float (&acc)[4][4], const floa
 // 1. near-DEM flag; far subjects get their porosity now.
 template <typename T>
 __global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T> gd, T csup,
-                           int *flag, int remap) {
+                           int *flag, int *dem_near, int remap) {
     const int k = subject_id(remap);
     if (k >= gs.n) return;
     const V4<T> me = gs.s[k];
@@ -311,7 +311,12 @@ __global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T>
         for (int s = lo; s < hi; ++s) {
             const V4<T> q = gd.s[s];
             const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
-            if (dist < (T)2.0 * D.h[gd.ids[s]]) near = true;
+            if (dist < (T)2.0 * D.h[gd.ids[s]]) {
+                near = true;
+                // the DEM's own fluid search (k_interaction_dem) tests the
+                // same distance of the same two stored positions
+                if (dem_near) dem_near[gd.ids[s]] = 1;
+            }
         }
     });
     flag[k] = near ? 1 : 0;
@@ -518,7 +523,8 @@ template <typename T> T sqrt_threshold(T s) {
 template <typename T>
 int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
                           const sphdem_grid_t *gs, const sphdem_grid_t *gd,
-                          const sphdem_work_t *w, double csup, double hu, hipStream_t st) {
+                          const sphdem_work_t *w, double csup, double hu, int *dem_near,
+                          hipStream_t st) {
     const int n = gs->n;
     if (n == 0) return SPHDEM_OK;
     ProfScope prof(SPHDEM_PASS_POROSITY, st);
@@ -526,7 +532,8 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
     const int bt = tuned_block(SPHDEM_PASS_POROSITY), rm = tuned_remap(SPHDEM_PASS_POROSITY);
     const int xt = tuning().por_x_tile;
     hipLaunchKernelGGL(k_por_mark<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
-                       view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), cs, w->fl_flag, rm);
+                       view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), cs, w->fl_flag, dem_near,
+                       rm);
     const T thr2 = sqrt_threshold<T>(cs);
     if (p->fast_math)
         hipLaunchKernelGGL((k_por_stream<T, true>), dim3(blocks_for(n, bt)), dim3(bt), 0, st,
@@ -541,10 +548,10 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
 template int launch_porosity_tiled<float>(const sphdem_params_t *, const sphdem_sph_t *,
                                           const sphdem_dem_t *, const sphdem_grid_t *,
                                           const sphdem_grid_t *, const sphdem_work_t *, double,
-                                          double, hipStream_t);
+                                          double, int *, hipStream_t);
 template int launch_porosity_tiled<double>(const sphdem_params_t *, const sphdem_sph_t *,
                                            const sphdem_dem_t *, const sphdem_grid_t *,
                                            const sphdem_grid_t *, const sphdem_work_t *, double,
-                                           double, hipStream_t);
+                                           double, int *, hipStream_t);
 
 }  // namespace sphdem

@@ -26,7 +26,7 @@ class Workspace:
     def __init__(self, device, dtype=torch.float32):
         self.device = torch.device(device)
         self.dtype = dtype
-        self.cap_f = self.cap_b = self.cap_scan = -1
+        self.cap_f = self.cap_b = self.cap_scan = self.cap_d = -1
         self.flags = torch.zeros(4, dtype=torch.int32, device=self.device)
         self.reserve(1, 1, 1)
 
@@ -55,13 +55,18 @@ class Workspace:
         if need > self.cap_scan:
             self.scan_tmp = torch.zeros(need, dtype=torch.int32, device=d)
             self.cap_scan = need
+        if nd > self.cap_d:
+            # zero between steps: the coupled step sets and clears the flags it uses
+            self.dem_near = torch.zeros(max(nd, 1), dtype=torch.int32, device=d)
+            self.cap_d = max(nd, 1)
 
     def struct(self) -> _lib.WorkStruct:
         w = _lib.WorkStruct()
         for name in ("fl_rho", "fl_pres", "fl_V", "fl_vel", "fl_pgf", "wl_rho", "wl_pres",
-                     "wl_vel", "scan_tmp", "flags", "fl_flag", "fl_list", "fl_count", "fl_scan"):
+                     "wl_vel", "scan_tmp", "flags", "fl_flag", "fl_list", "fl_count", "fl_scan",
+                     "dem_near"):
             setattr(w, name, getattr(self, name).data_ptr())
-        w.cap_f, w.cap_b = self.cap_f, self.cap_b
+        w.cap_f, w.cap_b, w.cap_d = self.cap_f, self.cap_b, self.cap_d
         return w
 
 
