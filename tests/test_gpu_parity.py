@@ -88,6 +88,24 @@ def test_neighbour_counts_bit_exact():
         np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("origin", [0.0, 1.9, -0.7])
+def test_neighbour_counts_on_the_support_boundary(origin):
+    """A lattice whose spacing divides the radius (pairs at exactly r, and at
+    sqrt(k) spacings around it), with points on bucket boundaries, near the
+    origin and far from it: the bucket walk counts every pair the oracle's
+    walk counts."""
+    d = 0.01
+    g = np.arange(12) * d + origin
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    r = 2 * d
+    for cell in (r, 1.5 * r, 0.9 * r):
+        os_, gs_ = _grid_pair(pts, cell, (64, 64, 64))
+        ref = O.neighbour_counts(os_, pts, os_, pts, r)
+        got = np_(neighbour_counts(gs_, gs_, r))
+        np.testing.assert_array_equal(got, ref)
+        assert ref.max() > ref.min()
+
+
 def test_exclusive_scan_matches_prefix_sum():
     rng = np.random.default_rng(2)
     for n in (1, 7, 2048, 2049, 70000, 300001):

@@ -287,6 +287,22 @@ __device__ __forceinline__ void query_runs(const GridV<T> &g, T px, T py, T pz, 
         for (int y = b.y0; y <= b.y1; ++y) row_runs(g, b, y, z, fn);
 }
 
+// A lower bound of |x - p| over the points x of virtual cell c along one
+// axis and the points p of [lo_p, hi_p].  Cell c holds int(x * inv) == c
+// (truncation toward zero, so cell 0 spans (-w, w)), w = 1 / inv.  The
+// bound is lowered by 1e-5 of the magnitudes involved, far more than the
+// rounding of x * inv, of w and of this arithmetic.
+template <typename T>
+__device__ __forceinline__ T axis_gap(T lo_p, T hi_p, int c, T w) {
+    const T lo = (T)(c <= 0 ? c - 1 : c) * w;
+    const T hi = (T)(c >= 0 ? c + 1 : c) * w;
+    T d = lo - hi_p;
+    const T e = lo_p - hi;
+    d = d > e ? d : e;
+    d = d - (T)1.0e-5 * (mabs(lo) + mabs(hi) + mabs(lo_p) + mabs(hi_p));
+    return d > (T)0 ? d : (T)0;
+}
+
 // Visit slots a, a+step, ... < b in order, loading the records of U slots
 // before evaluating any of them, so that their loads are in flight together
 // (the neighbour passes are bound by load latency); the evaluation order,

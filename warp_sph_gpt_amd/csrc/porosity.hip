@@ -62,19 +62,6 @@ template <typename T> __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
-// Conservative distance from coordinate p to the points of bucket
-// coordinate c along one axis (int(x * inv) == c, truncation toward zero:
-// c = 0 spans (-w, w)); w is slightly enlarged so rounding never culls.
-template <typename T>
-__device__ __forceinline__ T axis_gap(T lo_p, T hi_p, int c, T w) {
-    const T lo = (T)(c <= 0 ? c - 1 : c) * w;
-    const T hi = (T)(c >= 0 ? c + 1 : c) * w;
-    T d = lo - hi_p;
-    const T e = lo_p - hi;
-    d = d > e ? d : e;
-    return d > (T)0 ? d : (T)0;
-}
-
 // Slot ranges of virtual x cells [xa, xb] (xa + 2^20 >= 0, at most dx cells)
 // in a bucket row: two ranges when the row wraps around the folded grid.
 __device__ __forceinline__ int row_slots(const int *lo, const int *hi, int base, int dx, int xa,
@@ -325,8 +312,8 @@ __global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T>
 
 // Stream the union of the balls of the wave's lanes with `in` set; other
 // lanes compare against a zero threshold and add nothing.  Returns false
-// (having done nothing) if that union spans more buckets than the folded grid
-// holds along some axis.
+// (having done nothing) if that union spans more than dims - 2 buckets of
+// the folded grid along some axis.
 template <typename T, bool FAST>
 __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &gs, const T *Vs,
                              V4<T> *wbuf, T *vbuf, int lane, T csup, T thr2, T h, T inv_h) {
@@ -334,15 +321,17 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
     const T bxl = uniform_f(wave_min(in ? me.x : big)), bxh = uniform_f(wave_max(in ? me.x : -big));
     const T byl = uniform_f(wave_min(in ? me.y : big)), byh = uniform_f(wave_max(in ? me.y : -big));
     const T bzl = uniform_f(wave_min(in ? me.z : big)), bzh = uniform_f(wave_max(in ? me.z : -big));
-    const T inv = gs.inv, w = ((T)1.0 / inv) * (T)1.00001;
+    const T inv = gs.inv, w = (T)1.0 / inv;
     const T r2c = csup * csup * (T)1.0001;
     // union of the lanes' query boxes: int() is monotone, so the box of the
     // extreme positions bounds every lane's [int((p-r)/w), int((p+r)/w)]
     const int X0 = uniform_i((int)((bxl - csup) * inv)), X1 = uniform_i((int)((bxh + csup) * inv));
     const int Y0 = uniform_i((int)((byl - csup) * inv)), Y1 = uniform_i((int)((byh + csup) * inv));
     const int Z0 = uniform_i((int)((bzl - csup) * inv)), Z1 = uniform_i((int)((bzh + csup) * inv));
-    if (!(X1 - X0 < gs.dx && Y1 - Y0 < gs.dy && Z1 - Z0 < gs.dz && X0 + kOrigin >= 0 &&
-          Y0 + kOrigin >= 0 && Z0 + kOrigin >= 0))
+    // at most dims - 2 cells per axis, so every other fold of a bucket lies
+    // two cells outside the box and the culling below drops no term
+    if (!(X1 - X0 + 3 <= gs.dx && Y1 - Y0 + 3 <= gs.dy && Z1 - Z0 + 3 <= gs.dz &&
+          X0 + kOrigin >= 0 && Y0 + kOrigin >= 0 && Z0 + kOrigin >= 0))
         return false;
     const T thr = in ? thr2 : (T)0;
     // FAST: the subject relative to the group centre (far away if not in)
