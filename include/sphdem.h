@@ -48,7 +48,9 @@ typedef struct {
     double h_uniform;            /* search_scales[3]: common DEM h or 0 */
     int fast_math;               /* 0: W, dW/dr with IEEE division and square
                                     root, as the reference writes them;
-                                    1: hardware square root, 1/h multiplies */
+                                    1: in the porosity term and the force pass
+                                    the hardware square root and reciprocal,
+                                    1/h multiplies (densities stay exact) */
 } sphdem_params_t;
 
 /* input/struct.py:3-21 SPHptl */

@@ -148,7 +148,9 @@ class Scene:
         return errs
 
 
-def test_density_pressure_force_kernels():
+def test_density_pressure_force_kernels(math_mode):
+    """Shepard, fluid and wall density, EOS and force against the oracle,
+    with the reference's arithmetic and with the fast forms."""
     S = Scene()
     s = S.solv
     (P, B, D, W), (gP, gB, gD, gW) = S.o, S.g

@@ -384,7 +384,8 @@ _FAST_MATH = [False]
 def set_fast_math(flag: bool) -> None:
     """Default arithmetic of the neighbour sums: False evaluates W and dW/dr
     with IEEE division and square root exactly as the reference writes them;
-    True uses the hardware square root and multiplies by 1/h."""
+    True uses the hardware square root and reciprocal and multiplies by 1/h
+    in the porosity term and the force pass (densities stay exact)."""
     _FAST_MATH[0] = bool(flag)
 
 

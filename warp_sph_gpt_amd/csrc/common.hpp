@@ -79,6 +79,18 @@ __device__ __forceinline__ float mpow(float a, float b) {
 __device__ __forceinline__ double mpow(double a, double b) { return ::pow(a, b); }
 __device__ __forceinline__ float mabs(float x) { return fabsf(x); }
 __device__ __forceinline__ double mabs(double x) { return ::fabs(x); }
+// Fast-math forms: the hardware square root and reciprocal in FP32 (about
+// one ulp), the IEEE operations in FP64.
+template <typename T> __device__ __forceinline__ T fast_sqrt(T x);
+template <> __device__ __forceinline__ float fast_sqrt<float>(float x) {
+    return __builtin_amdgcn_sqrtf(x);
+}
+template <> __device__ __forceinline__ double fast_sqrt<double>(double x) { return ::sqrt(x); }
+template <typename T> __device__ __forceinline__ T fast_rcp(T x);
+template <> __device__ __forceinline__ float fast_rcp<float>(float x) {
+    return __builtin_amdgcn_rcpf(x);
+}
+template <> __device__ __forceinline__ double fast_rcp<double>(double x) { return 1.0 / x; }
 
 template <typename T> struct alignas(sizeof(T) * 4) V4 { T x, y, z, w; };
 template <typename T> struct V3 { T x, y, z; };

@@ -31,11 +31,6 @@
 
 namespace sphdem {
 
-template <typename T> __device__ __forceinline__ T fast_sqrt(T x);
-template <> __device__ __forceinline__ float fast_sqrt<float>(float x) {
-    return __builtin_amdgcn_sqrtf(x);
-}
-template <> __device__ __forceinline__ double fast_sqrt<double>(double x) { return ::sqrt(x); }
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float uniform_f(float v) {

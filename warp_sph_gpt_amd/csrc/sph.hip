@@ -19,6 +19,22 @@ namespace sphdem {
 
 constexpr int kThreads = 256;
 
+// The fast form of the Wendland C2 gradient (kernel/Kernel_KNL.py:26-41)
+// for r^2 < (2h)^2: (dW/dr) / r = c' u^3 with u = 1 - r/2h and
+// c' = -105 / (16 pi h^5), so no division by r is needed.
+template <typename T> struct FastDW {
+    T k1, c1;
+    __device__ explicit FastDW(T h) {
+        const T inv_h = (T)1.0 / h;
+        k1 = (T)-0.5 * inv_h;
+        c1 = (T)-105.0 / ((T)16.0 * (T)SPHDEM_PI * h * h * h) * inv_h * inv_h;
+    }
+    __device__ T over_r(T td) const {
+        const T u = fma(k1, fast_sqrt(td), (T)1.0);
+        return c1 * (u * u * u);
+    }
+};
+
 // Fluid subjects: Shepard filter and/or density (KERNEL_rho.py:6-82).
 //  SHEP: flt_i = sum (m_j/rho_j) W over fluid and wall, stored in P.flt.
 //  DENS: rho_i = sum m_j W / flt_i.
@@ -26,6 +42,8 @@ constexpr int kThreads = 256;
 //        sorted working arrays (P.rho is still read as the old density by
 //        the Shepard sum of other threads, so it is published by the force
 //        pass); P.pres is written here.  Without WORK density goes to P.rho.
+//  The density keeps the reference's arithmetic in both math modes: the
+//  Tait pressure multiplies its relative error by gamma B / p.
 template <typename T, bool SHEP, bool DENS, bool WORK>
 __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P, BndV<T> B,
                                                             GridV<T> gs, GridV<T> gb,
@@ -77,7 +95,8 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
 }
 
 // Wall subjects: raw density (KERNEL_rho.py:87-126); with WORK also the EOS
-// and the sorted copies the force pass reads.
+// and the sorted copies the force pass reads.  The reference's arithmetic in
+// both math modes, as for the fluid density.
 template <typename T, bool WORK>
 __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, BndV<T> B,
                                                            GridV<T> gs, GridV<T> gb, WorkV<T> w,
@@ -123,7 +142,10 @@ __global__ void k_pres(Prm<T> p, const T *rho, T *pres, int n) {
 // loops keep 1e-12 < r^2 < support^2 with the same support.
 //  PUBLISH: write the sorted density back to P.rho (fused step).
 //  PGF also leaves (pgf, rho) and m/rho in sorted order for the coupling.
-template <typename T, bool ACC, bool PGF, bool PUBLISH>
+//  FAST: dW/dr * rij / r is (C' / h) u^3 rij with u = 1 - r/2h (one hardware
+//  square root, no division by r), and the divisions by rho_j and by
+//  rho_i rho_j (r^2 + 0.01 h^2) are hardware reciprocals.
+template <typename T, bool ACC, bool PGF, bool PUBLISH, bool FAST>
 __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, BndV<T> B,
                                                           GridV<T> gs, GridV<T> gb, WorkV<T> w,
                                                           int remap) {
@@ -139,13 +161,27 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
     const T sup = p.support, ss = sup * sup, h = p.h, r2min = (T)1.0e-12;
     const T ai = presi / (rhoi * rhoi);
     const T two_mu = (T)2.0 * p.mu, reg = (T)0.01 * h * h;
+    const FastDW<T> fdw(h);
     V3<T> acc = mk3<T>(0, 0, 0), pgf = mk3<T>(0, 0, 0);
     auto body = [&](const GridV<T> &g, const T *nrho, const T *npres, const V4<T> *nvel) {
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
             each_slot<4>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int s, const V4<T> &q) {
                 const V3<T> rij = mk3(me.x - q.x, me.y - q.y, me.z - q.z);
                 const T td = dot(rij, rij);
-                if (td > r2min && td < ss) {
+                if constexpr (FAST) {
+                    if (td > r2min && td < ss) {
+                        const T gr = fdw.over_r(td);  // dW/dr / r
+                        const V3<T> dwij = gr * rij;
+                        const T rhoj = nrho[s], presj = npres[s];
+                        const T irho = fast_rcp(rhoj);
+                        if (ACC) {
+                            acc = acc - (q.w * (ai + presj * irho * irho)) * dwij;
+                            const T visc = two_mu * gr * td * fast_rcp(rhoi * rhoj * (td + reg));
+                            acc = acc + (q.w * visc) * (vi - xyz(nvel[s]));
+                        }
+                        if (PGF) pgf = pgf - ((q.w * irho) * (presj - presi)) * dwij;
+                    }
+                } else if (td > r2min && td < ss) {
                     const T r = msqrt(td);
                     const T dwv = wendland_dw(r, h);
                     const T rhoj = nrho[s], presj = npres[s];
@@ -279,9 +315,15 @@ static void launch_fluid_force(const sphdem_params_t *p, const sphdem_sph_t *P,
     if (gs->n == 0) return;
     ProfScope prof(SPHDEM_PASS_FLUID_FORCE, st);
     const int bt = tuned_block(SPHDEM_PASS_FLUID_FORCE);
-    hipLaunchKernelGGL((k_fluid_force<T, ACC, PGF, PUBLISH>), dim3(blocks_for(gs->n, bt)),
-                       dim3(bt), 0, st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs),
-                       view<T>(gb), view<T>(w), tuned_remap(SPHDEM_PASS_FLUID_FORCE));
+    const dim3 grid(blocks_for(gs->n, bt));
+    if (p->fast_math)
+        hipLaunchKernelGGL((k_fluid_force<T, ACC, PGF, PUBLISH, true>), grid, dim3(bt), 0, st,
+                           prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb),
+                           view<T>(w), tuned_remap(SPHDEM_PASS_FLUID_FORCE));
+    else
+        hipLaunchKernelGGL((k_fluid_force<T, ACC, PGF, PUBLISH, false>), grid, dim3(bt), 0, st,
+                           prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb),
+                           view<T>(w), tuned_remap(SPHDEM_PASS_FLUID_FORCE));
 }
 
 template <typename T>
