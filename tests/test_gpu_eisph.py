@@ -401,6 +401,27 @@ def test_steps_equal_repeated_one_steps(dtype, dx, fits, steps, diagnostics):
     assert torch.equal(L.part, L2.part)
 
 
+def test_steps_with_rows_longer_than_the_register_rows():
+    """Support 1.5x the cavity's (about 45 list entries a row): the FP32
+    one-point-per-thread kernel takes the entries beyond its register rows
+    from the list; 5 steps still equal 5 EISPH_OneStep calls bit for bit."""
+    gen, solv, P, B, _, _ = cavity(torch.float32, seed=8, dx=0.04, grid_slice=16)
+    solv.h = 1.5 * solv.h
+    P2, B2 = P.clone(), B.clone()
+    gs, gb = G.prepare(solv, P, B)
+    gs2, gb2 = G.prepare(solv, P2, B2)
+    L = G.neighbours(P, B, gs, gb, solv.support, solv.h)
+    assert G.one_group(P, B) and int(L.len.max()) > 24
+    for s in range(5):
+        G.EISPH_OneStep(solv, P, B, gs, gb, s == 4)
+    G.EISPH_Steps(solv, P2, B2, gs2, gb2, 5, True)
+    torch.cuda.synchronize()
+    for name in FIELDS:
+        assert torch.equal(getattr(P, name), getattr(P2, name)), name
+    for name in ("vel", "vel_star", "pres"):
+        assert torch.equal(getattr(B, name), getattr(B2, name)), f"ghost {name}"
+
+
 def test_steps_zero_is_a_no_op_and_negative_is_refused():
     gen, solv, P, B, _, _ = cavity(seed=3, dx=0.1, grid_slice=16)
     gs, gb = G.prepare(solv, P, B)

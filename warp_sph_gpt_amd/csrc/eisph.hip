@@ -461,6 +461,10 @@ template <typename T> long one_group_lds(int nf, int ng) {
     return (long)nf * (11 * (long)sizeof(T) + 2 * (long)sizeof(int)) +
            (long)ng * (3 * (long)sizeof(T) + (long)sizeof(int));
 }
+// ... and with the volume table of the register-row kernel
+template <typename T> long reg_group_lds(int nf, int ng) {
+    return one_group_lds<T>(nf, ng) + ((long)nf + ng) * (long)sizeof(T);
+}
 
 // Row entries from LDS-held (first slot, length), list loads of kU entries
 // issued before their terms are added in order.
@@ -485,15 +489,15 @@ __device__ __forceinline__ void row_batched(const NbrV<T> &L, int e, int len, Fn
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kOneThreads)
-k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, int nb,
-            int steps, int diag) {
-    extern __shared__ double one_lds[];
-    __shared__ double wsum[kOneThreads / 64];
-    __shared__ double gsum[kOneMaxGroups];
-    __shared__ T b0_s;
-    const int n = P.n, ng = B.n, tid = threadIdx.x;
+// The body of both one-workgroup kernels.  kReg > 0: one point per thread
+// (blockDim.x >= n), whose first kReg row entries stay in registers for the
+// whole launch, so the passes read only LDS (and the list for longer rows).
+template <typename T, int kReg>
+__device__ __forceinline__ void steps_body(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt,
+                                           double *part, int nb, int steps, int diag,
+                                           double *one_lds, double *wsum, double *gsum,
+                                           T &b0_s) {
+    const int n = P.n, ng = B.n, tid = threadIdx.x, nt = blockDim.x;
     constexpr int kWaves = kEisThreads / 64;  // waves of a five-launch workgroup
     T *vel = reinterpret_cast<T *>(one_lds);
     T *vst = vel + 3 * n;
@@ -503,10 +507,13 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
     T *A = bi + n;
     T *Ap = A + n;
     T *vbc = Ap + n;
-    int *mir = reinterpret_cast<int *>(vbc + 3 * ng);
+    // kReg > 0: V_j = m_j / rho_j of the fluid then the ghosts, the value
+    // the list's geometry holds (the same IEEE division as k_eis_fill)
+    T *vtab = vbc + 3 * ng;
+    int *mir = reinterpret_cast<int *>(vtab + (kReg > 0 ? n + ng : 0));
     int *ent = mir + ng;
     int *cnt = ent + n;
-    for (int i = tid; i < n; i += kOneThreads) {
+    for (int i = tid; i < n; i += nt) {
         st3(vel, i, ld3(P.vel, i));
         st3(vst, i, ld3(P.vel_star, i));
         pres[i] = P.pres[i];
@@ -514,21 +521,66 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
         ent[i] = slot0(L.off, i);
         cnt[i] = L.len[i];
     }
-    for (int b = tid; b < ng; b += kOneThreads) {
+    for (int b = tid; b < ng; b += nt) {
         st3(vbc, b, ld3(B.vel_bc, b));
         mir[b] = B.mirror[b];
     }
+    if (kReg > 0) {
+        for (int i = tid; i < n; i += nt) vtab[i] = P.m[i] / P.rho[i];
+        for (int b = tid; b < ng; b += nt) vtab[n + b] = B.m[b] / B.rho[b];
+    }
     __syncthreads();
     auto ghost = [&](const T *f, int b) { return ghost_of(ld3(vbc, b), ld3(f, mir[b])); };
+    // the thread's own row in registers (kReg > 0: point tid): id, gx, gz,
+    // A_ij of the first kReg entries; V_j comes from vtab
+    constexpr int kC = kReg > 0 ? kReg : 1;
+    int cid[kC];
+    T cgx[kC], cgz[kC], cA[kC];
+    int own_e = 0, own_len = 0;
+    if (kReg > 0 && tid < n) {
+        own_e = slot0(L.off, tid);
+        own_len = L.len[tid];
+#pragma unroll
+        for (int k = 0; k < kC; ++k) {
+            cid[k] = 0;
+            cgx[k] = cgz[k] = cA[k] = (T)0;
+            if (k < own_len) {
+                const V4<T> G = L.geo[own_e + k * kSlice];
+                cid[k] = L.id[own_e + k * kSlice];
+                cgx[k] = G.x;
+                cgz[k] = G.y;
+                cA[k] = G.z;
+            }
+        }
+    }
+    // row i's entries in order (with kReg > 0, i is always tid)
+    auto row = [&](int i, auto &&fn) {
+        if constexpr (kReg > 0) {
+#pragma unroll
+            for (int k = 0; k < kC; ++k) {
+                if (k < own_len) {
+                    const int j = cid[k];
+                    fn(j, V4<T>{cgx[k], cgz[k], cA[k], vtab[j >= 0 ? j : n - 1 - j]});
+                }
+                // keep the scheduler from hoisting every entry's LDS reads
+                // at once (their live ranges would spill the cached row)
+                if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+            for (int k = kC, e = own_e + kC * kSlice; k < own_len; ++k, e += kSlice)
+                fn(L.id[e], L.geo[e]);
+        } else {
+            row_batched(L, ent[i], cnt[i], fn);
+        }
+    };
 
     double total = 0.0;
     for (int s = 0; s < steps; ++s) {
         const bool last = s == steps - 1;
         // advec (KERNEL_EISPH_force.py:12-72)
-        for (int i = tid; i < n; i += kOneThreads) {
+        for (int i = tid; i < n; i += nt) {
             const V3<T> vi = ld3(vel, i);
             V3<T> conv = mk3((T)0, (T)0, (T)0), lap = conv;
-            row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+            row(i, [&](int j, const V4<T> &G) {
                 const V3<T> vj = j >= 0 ? ld3(vel, j) : ghost(vel, -1 - j);
                 advec_term(conv, lap, vi, vj, G);
             });
@@ -536,13 +588,13 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
         }
         __syncthreads();
         // ppe (KERNEL_EISPH_PPE.py:12-79) over the five-launch thread numbering
-        for (int t0 = 0; t0 < nb * kEisThreads; t0 += kOneThreads) {
+        for (int t0 = 0; t0 < nb * kEisThreads; t0 += nt) {
             const int i = t0 + tid;
             double mine = 0.0;
             if (i < n) {
                 const V3<T> vi = ld3(vst, i);
                 T div = 0, a = 0, ap = 0;
-                row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+                row(i, [&](int j, const V4<T> &G) {
                     V3<T> vj;
                     T pj;
                     if (j >= 0) {
@@ -566,7 +618,7 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
             const double w = wave_sum(mine);
             if ((tid & 63) == 0) wsum[tid >> 6] = w;
             __syncthreads();
-            if (tid < kOneThreads / kEisThreads) {
+            if (tid < nt / kEisThreads) {
                 const int g = t0 / kEisThreads + tid;
                 if (g < nb) {
                     double acc = 0.0;
@@ -591,7 +643,7 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
         // solve (KERNEL_EISPH_PPE.py:82-125)
         const T mean = (T)total / (T)n;
         const T gauge = gauge_of(b0_s, mean, A[0], Ap[0]);
-        for (int i = tid; i < n; i += kOneThreads) {
+        for (int i = tid; i < n; i += nt) {
             const T b = bi[i] - mean;
             bi[i] = b;
             pres[i] = solved_p(b, pres[i], A[i], Ap[i], gauge);
@@ -608,10 +660,10 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
         }
         __syncthreads();
         // correct (KERNEL_EISPH_step.py:12-61)
-        for (int i = tid; i < n; i += kOneThreads) {
+        for (int i = tid; i < n; i += nt) {
             const T pi = pres[i];
             V3<T> gp = mk3((T)0, (T)0, (T)0);
-            row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+            row(i, [&](int j, const V4<T> &G) {
                 correct_term(gp, pi, j >= 0 ? pres[j] : pres[mir[-1 - j]], G);
             });
             const V3<T> old = ld3(vel, i);
@@ -623,11 +675,11 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
     }
     // diagnostics of the last step (KERNEL_EISPH_step.py:64-121)
     if (diag) {
-        for (int i = tid; i < n; i += kOneThreads) {
+        for (int i = tid; i < n; i += nt) {
             const V3<T> vi = ld3(vel, i);
             const T pi = pres[i];
             T div = 0, op = 0;
-            row_batched(L, ent[i], cnt[i], [&](int j, const V4<T> &G) {
+            row(i, [&](int j, const V4<T> &G) {
                 V3<T> vj;
                 T pj;
                 if (j >= 0) {
@@ -645,18 +697,45 @@ k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, i
             P.res[i] = op - bi[i];
         }
     }
-    for (int i = tid; i < n; i += kOneThreads) {
+    for (int i = tid; i < n; i += nt) {
         st3(P.vel, i, ld3(vel, i));
         st3(P.vel_star, i, ld3(vst, i));
         P.pres[i] = pres[i];
     }
     // KERNEL_EISPH_BC.py:6-47 from the final fields
-    for (int b = tid; b < ng; b += kOneThreads) {
+    for (int b = tid; b < ng; b += nt) {
         st3(B.vel_star, b, ghost(vst, b));
         B.pres[b] = pres[mir[b]];
         st3(B.vel, b, ghost(vel, b));
     }
     if (tid < nb) part[2 + tid] = gsum[tid];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kOneThreads)
+k_eis_steps(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, int nb,
+            int steps, int diag) {
+    extern __shared__ double one_lds[];
+    __shared__ double wsum[kOneThreads / 64];
+    __shared__ double gsum[kOneMaxGroups];
+    __shared__ T b0_s;
+    steps_body<T, 0>(P, B, L, nu, rho0, dt, part, nb, steps, diag, one_lds, wsum, gsum, b0_s);
+}
+
+// FP32 clouds of at most kRegThreads points: one point per thread, rows of up
+// to kRegRow entries held in registers.
+constexpr int kRegThreads = 768;
+constexpr int kRegRow = 20;
+template <typename T>
+__global__ void __launch_bounds__(kRegThreads)
+k_eis_steps_reg(EisV<T> P, EisV<T> B, NbrV<T> L, T nu, T rho0, T dt, double *part, int nb,
+                int steps, int diag) {
+    extern __shared__ double one_lds[];
+    __shared__ double wsum[kRegThreads / 64];
+    __shared__ double gsum[kOneMaxGroups];
+    __shared__ T b0_s;
+    steps_body<T, kRegRow>(P, B, L, nu, rho0, dt, part, nb, steps, diag, one_lds, wsum, gsum,
+                           b0_s);
 }
 
 // ---------------------------------------------------------------- checks
@@ -969,6 +1048,16 @@ int sphdem_eisph_steps(const sphdem_eisph_params_t *p, sphdem_eisph_t *P, sphdem
     hipStream_t st = as_stream(stream);
     const int nb = ppe_groups(P->n, B->n);
     ProfScope ps(SPHDEM_PASS_EISPH_STEPS, st);
+    if (p->precision == 4 && P->n <= kRegThreads &&
+        reg_group_lds<float>(P->n, B->n) <= kOneMaxLds) {
+        // one point per thread, rows in registers
+        const int nt = kEisThreads * blocks_for(P->n, kEisThreads);
+        hipLaunchKernelGGL(k_eis_steps_reg<float>, dim3(1), dim3(nt),
+                           (size_t)reg_group_lds<float>(P->n, B->n), st, eview<float>(P),
+                           eview<float>(B), nview<float>(L), (float)p->nu, (float)p->rho0,
+                           (float)p->dt, L->part, nb, steps, diagnostics ? 1 : 0);
+        return check_launch("eisph_steps");
+    }
     EIS_DISPATCH(p->precision,
                  hipLaunchKernelGGL(k_eis_steps<T>, dim3(1), dim3(kOneThreads),
                                     (size_t)one_group_lds<T>(P->n, B->n), st, eview<T>(P),
