@@ -133,6 +133,13 @@ typedef struct {
      * search of the other DEMs (it would find no term) and clears the flags
      * it read.  NULL (or cap_d below the DEM count): no skipping. */
     int *dem_near;
+    /* [cap_b] or NULL: for callers whose walls never move (an engine that
+     * builds the wall grid once).  Per wall slot the density sum over wall
+     * neighbours alone, NaN-filled by the caller and whenever the wall grid,
+     * B->pos or B->m change.  A wall particle with no fluid term takes its
+     * density from here (the same bits), and the pass fills a NaN on first
+     * use.  NULL: every wall sum is evaluated. */
+    void *wl_static;
     int cap_f, cap_b, cap_d;
 } sphdem_work_t;
 

@@ -175,3 +175,28 @@ def test_dem_near_flags_skip_only_dry_dems(fast_math):
         assert torch.equal(getattr(flagged.D, name), getattr(plain.D, name)), name
     dry = np_(flagged.D.flt_s) == 0
     assert dry.any() and not dry.all()
+
+
+def test_static_wall_densities_change_nothing():
+    """Wall particles with no fluid in their support take their density from
+    the engine's cache of wall-only sums: 40 steps of the shipped dam break
+    with the cache and without it leave every field equal bit for bit, wall
+    densities and pressures included; walls_changed() forgets the cache."""
+    wl = c0(device=DEV)
+    cached = CoupledEngine(wl.solv, *[s.to(DEV).clone() for s in wl.structs])
+    plain = CoupledEngine(wl.solv, *[s.to(DEV).clone() for s in wl.structs])
+    plain.ws.wl_static = None
+    for _ in range(40):
+        cached.step()
+        plain.step()
+    torch.cuda.synchronize()
+    for name in ("pos", "vel", "acc", "rho", "pres", "flt"):
+        assert torch.equal(getattr(cached.P, name), getattr(plain.P, name)), name
+    for name in ("rho", "pres"):
+        assert torch.equal(getattr(cached.B, name), getattr(plain.B, name)), f"wall {name}"
+    for name in ("pos", "vel", "omega", "force", "drag"):
+        assert torch.equal(getattr(cached.D, name), getattr(plain.D, name)), name
+    known = torch.isfinite(cached.ws.wl_static[:cached.B.n])
+    assert 0 < int(known.sum()) < cached.B.n
+    cached.walls_changed()
+    assert not bool(torch.isfinite(cached.ws.wl_static).any())

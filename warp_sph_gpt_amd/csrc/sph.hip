@@ -97,10 +97,15 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
 // Wall subjects: raw density (KERNEL_rho.py:87-126); with WORK also the EOS
 // and the sorted copies the force pass reads.  The reference's arithmetic in
 // both math modes, as for the fluid density.
+//  wall_only (or null): per wall slot, the sum over wall neighbours alone,
+//  NaN until first needed.  A subject whose fluid sum is exactly +0 then has
+//  rho = 0 + (the same wall terms in the same order), a function of the
+//  static wall positions and masses, so it is taken from there (and stored
+//  on first use).  Valid while the wall grid, B.pos and B.m are unchanged.
 template <typename T, bool WORK>
 __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, BndV<T> B,
                                                            GridV<T> gs, GridV<T> gb, WorkV<T> w,
-                                                           int remap) {
+                                                           T *wall_only, int remap) {
     const int k = subject_id(remap);
     if (k >= gb.n) return;
     const V4<T> me = gb.s[k];
@@ -120,7 +125,17 @@ __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, 
         });
     };
     acc(gs);
-    acc(gb);
+    if (wall_only && rho == (T)0) {
+        const T c = wall_only[k];
+        if (c == c) {
+            rho = c;
+        } else {
+            acc(gb);
+            wall_only[k] = rho;
+        }
+    } else {
+        acc(gb);
+    }
     B.rho[bi] = rho;
     if (WORK) {
         const T pr = tait(rho, p.rho0, p.c0, p.gamma);
@@ -303,9 +318,10 @@ static void launch_wall_density(const sphdem_params_t *p, const sphdem_sph_t *P,
     WorkV<T> wv{};
     if (w) wv = view<T>(w);
     const int bt = tuned_block(SPHDEM_PASS_WALL_DENSITY);
+    T *wall_only = (w && w->wl_static && w->cap_b >= gb->n) ? (T *)w->wl_static : nullptr;
     hipLaunchKernelGGL((k_wall_density<T, WORK>), dim3(blocks_for(gb->n, bt)), dim3(bt), 0, st,
                        prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb), wv,
-                       tuned_remap(SPHDEM_PASS_WALL_DENSITY));
+                       wall_only, tuned_remap(SPHDEM_PASS_WALL_DENSITY));
 }
 
 template <typename T, bool ACC, bool PGF, bool PUBLISH>

@@ -48,6 +48,8 @@ class CoupledEngine:
         self.ws = Workspace(dev, dtype)
         nd = P_dem.n if self.coupled else 0
         self.ws.reserve(P_sph.n, P_bnd.n, nd)
+        # the wall grid is built once above: walls never move in an engine
+        self.ws.static_walls()
         self.scales = None
         if self.coupled:
             self.scales = validate_dem_particles(solv, P_dem, P_dem_bnd)
@@ -129,6 +131,13 @@ class CoupledEngine:
     def run(self, n_steps: int) -> None:
         for _ in range(int(n_steps)):
             self.step()
+
+    def walls_changed(self) -> None:
+        """After editing the wall positions or masses in place: rebuild the
+        wall grid and forget the cached wall-only densities (in place, so
+        captured graphs stay valid)."""
+        self.gb.build(self.B.pos, self.solv.support, weights=self.B.m)
+        self.ws.wl_static.fill_(float("nan"))
 
     def check(self) -> None:
         """Raise OverflowError if a contact row did not fit the capacity."""

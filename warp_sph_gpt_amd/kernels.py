@@ -27,6 +27,7 @@ class Workspace:
         self.device = torch.device(device)
         self.dtype = dtype
         self.cap_f = self.cap_b = self.cap_scan = self.cap_d = -1
+        self.wl_static = None   # see static_walls()
         self.flags = torch.zeros(4, dtype=torch.int32, device=self.device)
         self.reserve(1, 1, 1)
 
@@ -51,6 +52,8 @@ class Workspace:
             self.wl_pres = torch.zeros(c, dtype=t, device=d)
             self.wl_vel = torch.zeros((c, 4), dtype=t, device=d)
             self.cap_b = c
+            if self.wl_static is not None:
+                self.static_walls()
         need = _lib.load().sphdem_scan_scratch(max(nd, 1) + 1)
         if need > self.cap_scan:
             self.scan_tmp = torch.zeros(need, dtype=torch.int32, device=d)
@@ -60,12 +63,21 @@ class Workspace:
             self.dem_near = torch.zeros(max(nd, 1), dtype=torch.int32, device=d)
             self.cap_d = max(nd, 1)
 
+    def static_walls(self) -> None:
+        """Keep the wall-only density sums of wall particles with no fluid
+        near (sphdem_work_t.wl_static) for walls that never move: only for
+        an owner that builds its wall grid once (an engine).  Call again
+        after changing the wall grid, positions or masses."""
+        self.wl_static = torch.full((max(self.cap_b, 1),), float("nan"), dtype=self.dtype,
+                                    device=self.device)
+
     def struct(self) -> _lib.WorkStruct:
         w = _lib.WorkStruct()
         for name in ("fl_rho", "fl_pres", "fl_V", "fl_vel", "fl_pgf", "wl_rho", "wl_pres",
                      "wl_vel", "scan_tmp", "flags", "fl_flag", "fl_list", "fl_count", "fl_scan",
                      "dem_near"):
             setattr(w, name, getattr(self, name).data_ptr())
+        w.wl_static = self.wl_static.data_ptr() if self.wl_static is not None else None
         w.cap_f, w.cap_b, w.cap_d = self.cap_f, self.cap_b, self.cap_d
         return w
 
