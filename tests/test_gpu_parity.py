@@ -88,6 +88,26 @@ def test_neighbour_counts_bit_exact():
         np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("h", [0.026, 0.0052, 0.052 / 3])
+def test_wendland_kernels_bit_exact_against_the_oracle(h):
+    """W and dW/dr as the SPH passes evaluate them (the divisions by h,
+    16 pi h^3 and 16 pi h^4 through reciprocals held in double) equal the
+    oracle's IEEE FP32 evaluation bit for bit: r = 0, random radii over
+    [0, 2.2h], and runs of consecutive floats at 1e-6 h, h and 2h (a square
+    root of a float is 0 or normal, so no subnormal radius occurs there)."""
+    rng = np.random.default_rng(3)
+    h32 = np.float32(h)
+    parts = [np.zeros(1, np.float32), (rng.random(300_000) * 2.2 * h).astype(np.float32)]
+    for c in (np.float32(1e-6) * h32, h32, np.float32(2) * h32):
+        bits = np.array([c], np.float32).view(np.int32).astype(np.int64)
+        parts.append((bits + np.arange(-20_000, 20_000)).astype(np.int32).view(np.float32))
+    r = np.concatenate(parts)
+    w_ref, dw_ref = O.wendland(r, float(h32))
+    w, dw = K.wendland(torch.tensor(r, device=DEV), float(h32))
+    np.testing.assert_array_equal(np_(w).view(np.uint32), w_ref.view(np.uint32))
+    np.testing.assert_array_equal(np_(dw).view(np.uint32), dw_ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("origin", [0.0, 1.9, -0.7])
 def test_neighbour_counts_on_the_support_boundary(origin):
     """A lattice whose spacing divides the radius (pairs at exactly r, and at

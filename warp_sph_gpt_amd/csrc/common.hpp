@@ -143,6 +143,48 @@ template <typename T> __device__ __forceinline__ T wendland_dw(T r, T h) {
     }
     return dw;
 }
+// The same two functions with their divisions by the pass-wide constants h,
+// 16 pi h^3 and 16 pi h^4 done through reciprocals held in double.  For FP32
+// a and b, (float)((double)a * RN64(1/b)) is the correctly rounded a/b: an
+// exact float quotient is never a midpoint between floats and lies at least
+// 2^-50 (relative) from one, while the double product is within 2^-52 of a/b
+// (normal quotients; these passes make no subnormal ones).  Three full-rate
+// instructions replace a division sequence; the bits are the IEEE
+// division's.  FP64 keeps the IEEE division.
+template <typename T> struct WendlandK {
+    T h, c3, c4;
+    __device__ explicit WendlandK(T hh)
+        : h(hh), c3((T)16.0 * (T)SPHDEM_PI * hh * hh * hh),
+          c4((T)16.0 * (T)SPHDEM_PI * hh * hh * hh * hh) {}
+    __device__ T div_h(T a) const { return a / h; }
+    __device__ T div_c3(T a) const { return a / c3; }
+    __device__ T div_c4(T a) const { return a / c4; }
+};
+template <> struct WendlandK<float> {
+    double ih, ic3, ic4;
+    __device__ explicit WendlandK(float h)
+        : ih(1.0 / (double)h), ic3(1.0 / (double)(16.0f * (float)SPHDEM_PI * h * h * h)),
+          ic4(1.0 / (double)(16.0f * (float)SPHDEM_PI * h * h * h * h)) {}
+    __device__ float div_h(float a) const { return (float)((double)a * ih); }
+    __device__ float div_c3(float a) const { return (float)((double)a * ic3); }
+    __device__ float div_c4(float a) const { return (float)((double)a * ic4); }
+};
+template <typename T> __device__ __forceinline__ T wendland_w(T r, const WendlandK<T> &k) {
+    T q = k.div_h(r);
+    T w = (T)0;
+    T u = (T)1.0 - (T)0.5 * q;
+    if (q < (T)2.0) w = k.div_c3((T)21.0 * u * u * u * u * ((T)1.0 + (T)2.0 * q));
+    return w;
+}
+template <typename T> __device__ __forceinline__ T wendland_dw(T r, const WendlandK<T> &k) {
+    T q = k.div_h(r);
+    T dw = (T)0;
+    if (q < (T)2.0) {
+        T u = (T)1.0 - (T)0.5 * q;
+        dw = k.div_c4((T)21.0 * ((T)-5.0 * q * u * u * u));
+    }
+    return dw;
+}
 // kernel/KERNEL_pres.py:21-23
 template <typename T> __device__ __forceinline__ T tait(T rho, T rho0, T c0, T gamma) {
     const T B = rho0 * c0 * c0 / gamma;

@@ -56,13 +56,14 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
     gs = at_inst(gs, ik);
     gb = at_inst(gb, ik);
     const T sup = p.support, ss = sup * sup, h = p.h;
+    const WendlandK<T> wk(h);
     T flt = (T)0, rho = (T)0;
     query_runs(gs, me.x, me.y, me.z, sup, [&](int a, int b) {
         each_slot<4>(a, b, 1, [&](int s) { return gs.s[s]; }, [&](int s, const V4<T> &q) {
             const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
             const T td = dx * dx + dy * dy + dz * dz;
             if (td < ss) {
-                const T wv = wendland_w(msqrt(td), h);
+                const T wv = wendland_w(msqrt(td), wk);
                 if (SHEP) flt = flt + (q.w / P.rho[gs.ids[s]]) * wv;
                 if (DENS) rho = rho + q.w * wv;
             }
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
             const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
             const T td = dx * dx + dy * dy + dz * dz;
             if (td < ss) {
-                const T wv = wendland_w(msqrt(td), h);
+                const T wv = wendland_w(msqrt(td), wk);
                 if (SHEP) flt = flt + (q.w / B.rho[gb.ids[s]]) * wv;
                 if (DENS) rho = rho + q.w * wv;
             }
@@ -114,13 +115,14 @@ __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, 
     gs = at_inst(gs, ik);
     gb = at_inst(gb, ik);
     const T sup = p.support, ss = sup * sup, h = p.h;
+    const WendlandK<T> wk(h);
     T rho = (T)0;
     auto acc = [&](const GridV<T> &g) {
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
             each_slot<4>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int, const V4<T> &q) {
                 const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
                 const T td = dx * dx + dy * dy + dz * dz;
-                if (td < ss) rho = rho + q.w * wendland_w(msqrt(td), h);
+                if (td < ss) rho = rho + q.w * wendland_w(msqrt(td), wk);
             });
         });
     };
@@ -177,6 +179,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
     const T ai = presi / (rhoi * rhoi);
     const T two_mu = (T)2.0 * p.mu, reg = (T)0.01 * h * h;
     const FastDW<T> fdw(h);
+    const WendlandK<T> wk(h);
     V3<T> acc = mk3<T>(0, 0, 0), pgf = mk3<T>(0, 0, 0);
     auto body = [&](const GridV<T> &g, const T *nrho, const T *npres, const V4<T> *nvel) {
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
                     }
                 } else if (td > r2min && td < ss) {
                     const T r = msqrt(td);
-                    const T dwv = wendland_dw(r, h);
+                    const T dwv = wendland_dw(r, wk);
                     const T rhoj = nrho[s], presj = npres[s];
                     if (ACC) {
                         const V3<T> dwij = dwv * (rij / r);
@@ -264,8 +267,9 @@ template <typename T>
 __global__ void k_wendland(const T *r, int n, T h, T *w, T *dw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    w[i] = wendland_w(r[i], h);
-    dw[i] = wendland_dw(r[i], h);
+    const WendlandK<T> wk(h);
+    w[i] = wendland_w(r[i], wk);
+    dw[i] = wendland_dw(r[i], wk);
 }
 
 // ---------------------------------------------------------------- host side
