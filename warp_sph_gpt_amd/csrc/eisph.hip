@@ -54,12 +54,33 @@ __device__ __forceinline__ int slot0(const int *off, int i) {
     return off[i / kSlice] + (i % kSlice);
 }
 
+// The entries of a row from its first slot and length, the list loads of
+// kU entries issued before their terms are added in order.
+template <typename T, typename Fn>
+__device__ __forceinline__ void row_batched(const NbrV<T> &L, int e, int len, Fn &&fn) {
+    constexpr int kU = 4;
+    for (int k = 0; k < len; k += kU, e += kU * kSlice) {
+        int id[kU];
+        V4<T> g[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            id[u] = 0;
+            g[u] = V4<T>{(T)0, (T)0, (T)0, (T)0};
+            if (k + u < len) {
+                id[u] = L.id[e + u * kSlice];
+                g[u] = L.geo[e + u * kSlice];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (k + u < len) fn(id[u], g[u]);
+    }
+}
+
 // Visit row i's entries in order: fn(id, geo).
 template <typename T, typename Fn>
 __device__ __forceinline__ void each_entry(const NbrV<T> &L, int i, Fn &&fn) {
-    const int n = L.len[i];
-    int e = slot0(L.off, i);
-    for (int k = 0; k < n; ++k, e += kSlice) fn(L.id[e], L.geo[e]);
+    row_batched(L, slot0(L.off, i), L.len[i], fn);
 }
 
 // ---------------------------------------------------------------- kernels of the pair
@@ -466,28 +487,6 @@ template <typename T> long reg_group_lds(int nf, int ng) {
     return one_group_lds<T>(nf, ng) + ((long)nf + ng) * (long)sizeof(T);
 }
 
-// Row entries from LDS-held (first slot, length), list loads of kU entries
-// issued before their terms are added in order.
-template <typename T, typename Fn>
-__device__ __forceinline__ void row_batched(const NbrV<T> &L, int e, int len, Fn &&fn) {
-    constexpr int kU = 4;
-    for (int k = 0; k < len; k += kU, e += kU * kSlice) {
-        int id[kU];
-        V4<T> g[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            id[u] = 0;
-            g[u] = V4<T>{(T)0, (T)0, (T)0, (T)0};
-            if (k + u < len) {
-                id[u] = L.id[e + u * kSlice];
-                g[u] = L.geo[e + u * kSlice];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u)
-            if (k + u < len) fn(id[u], g[u]);
-    }
-}
 
 // The body of both one-workgroup kernels.  kReg > 0: one point per thread
 // (blockDim.x >= n), whose first kReg row entries stay in registers for the
