@@ -184,21 +184,23 @@ __global__ void k_grid_scatter(const int *cells, const int *rank, const int *lo,
     if (i < n) ids[lo[cells[i]] + rank[i]] = i;
 }
 
+// Warp's order inside a bucket is ascending point index.  The scatter left
+// each bucket's ids in atomic-arrival order in `tmp`; point i takes the slot
+// after the ids of its bucket smaller than i (cell_hi still holds counts).
+__global__ void k_grid_order(const int *cells, const int *lo, const int *cnt, const int *tmp,
+                             int n, int *ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cells[i], a = lo[c], e = a + cnt[c];
+    int pos = a;
+    for (int k = a; k < e; ++k) pos += tmp[k] < i;
+    ids[pos] = i;
+}
+
 // cell_hi holds the counts on entry and the bucket ends on exit.
-__global__ void k_grid_fix(const int *lo, int *hi, int *ids, int ncell) {
+__global__ void k_grid_fix(const int *lo, int *hi, int ncell) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int a = lo[c], cnt = hi[c];
-    hi[c] = a + cnt;
-    for (int k = a + 1; k < a + cnt; ++k) {
-        const int v = ids[k];
-        int j = k - 1;
-        while (j >= a && ids[j] > v) {
-            ids[j + 1] = ids[j];
-            --j;
-        }
-        ids[j + 1] = v;
-    }
+    if (c < ncell) hi[c] = lo[c] + hi[c];
 }
 
 template <typename T>
@@ -226,11 +228,17 @@ static int grid_build(sphdem_grid_t *g, const void *points, int n, double radius
                            g->point_cells, g->rank, g->cell_hi);
     int rc = exclusive_scan(g->cell_hi, g->cell_lo, ncell, g->scan_tmp, st);
     if (rc) return rc;
-    if (n > 0)
+    if (n > 0) {
+        // the sorted copy's storage (4n values, written last) holds the
+        // buckets in arrival order meanwhile
+        int *tmp = reinterpret_cast<int *>(g->sorted);
         hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
-                           g->point_cells, g->rank, g->cell_lo, n, g->point_ids);
+                           g->point_cells, g->rank, g->cell_lo, n, tmp);
+        hipLaunchKernelGGL(k_grid_order, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
+                           g->point_cells, g->cell_lo, g->cell_hi, tmp, n, g->point_ids);
+    }
     hipLaunchKernelGGL(k_grid_fix, dim3(blocks_for(ncell, bt)), dim3(bt), 0, st, g->cell_lo,
-                       g->cell_hi, g->point_ids, ncell);
+                       g->cell_hi, ncell);
     if (n > 0)
         hipLaunchKernelGGL(k_grid_sorted<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            (const T *)points, (const T *)w, g->point_ids, n, (V4<T> *)g->sorted);
