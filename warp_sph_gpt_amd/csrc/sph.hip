@@ -159,9 +159,9 @@ __global__ void k_pres(Prm<T> p, const T *rho, T *pres, int n) {
 // loops keep 1e-12 < r^2 < support^2 with the same support.
 //  PUBLISH: write the sorted density back to P.rho (fused step).
 //  PGF also leaves (pgf, rho) and m/rho in sorted order for the coupling.
-//  FAST: dW/dr * rij / r is (C' / h) u^3 rij with u = 1 - r/2h (one hardware
-//  square root, no division by r), and the divisions by rho_j and by
-//  rho_i rho_j (r^2 + 0.01 h^2) are hardware reciprocals.
+//  FAST: dW/dr * rij / r is c' u^3 rij with u = 1 - r/2h (FastDW: one
+//  hardware square root, no division by r), and the divisions by rho_j and
+//  by rho_i rho_j (r^2 + 0.01 h^2) are hardware reciprocals.
 template <typename T, bool ACC, bool PGF, bool PUBLISH, bool FAST>
 __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, BndV<T> B,
                                                           GridV<T> gs, GridV<T> gb, WorkV<T> w,
