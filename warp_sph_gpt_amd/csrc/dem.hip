@@ -376,9 +376,13 @@ __global__ __launch_bounds__(kDemThreads) void k_interaction_dem(Prm<T> p, DemV<
 // kernel/KERNEL_SPHDEM_interaction.py:190-226 (INTER) fused with
 // kernel/KERNEL_step.py:6-22 (STEP): both only touch the subject's own fields.
 template <typename T, bool INTER, bool STEP>
+// near (or null): the porosity mark's near-DEM flag of each sorted fluid
+// slot, from this step's positions.  The mark ran this very DEM query and
+// tested the same distance, so an unflagged subject has no term here and
+// skips the query (fd stays +0, as its loop would leave it).
 __global__ __launch_bounds__(kThreads) void k_fluid_final(Prm<T> p, SphV<T> P, DemV<T> D,
                                                           GridV<T> gs, GridV<T> gd, T csup,
-                                                          int remap) {
+                                                          const int *near, int remap) {
     const int k = subject_id(remap);
     if (k >= gs.n) return;
     const int i = gs.ids[k];
@@ -387,7 +391,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_final(Prm<T> p, SphV<T> P, D
     if (INTER) {
         const V4<T> me = gs.s[k];
         V3<T> fd = mk3<T>(0, 0, 0);
-        query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+        if (!near || near[k]) query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
             for (int s = lo; s < hi; ++s) {
                 const int b = gd.ids[s];
                 const V4<T> q = gd.s[s];
@@ -579,13 +583,14 @@ static void launch_interaction_dem(const sphdem_params_t *p, const sphdem_dem_t 
 template <typename T, bool INTER, bool STEP>
 static void launch_fluid_final(const sphdem_params_t *p, const sphdem_sph_t *P,
                                const sphdem_dem_t *D, const sphdem_grid_t *gs,
-                               const sphdem_grid_t *gd, double csup, hipStream_t st) {
+                               const sphdem_grid_t *gd, double csup, hipStream_t st,
+                               const int *near = nullptr) {
     if (gs->n == 0) return;
     ProfScope prof(SPHDEM_PASS_FLUID_FINAL, st);
     const int bt = tuned_block(SPHDEM_PASS_FLUID_FINAL);
     hipLaunchKernelGGL((k_fluid_final<T, INTER, STEP>), dim3(blocks_for(gs->n, bt)), dim3(bt), 0,
                        st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                       (T)csup, tuned_remap(SPHDEM_PASS_FLUID_FINAL));
+                       (T)csup, near, tuned_remap(SPHDEM_PASS_FLUID_FINAL));
 }
 
 template <typename T>
@@ -637,7 +642,8 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near);
         if (rc) return rc;
         launch_interaction_dem<T>(p, D, gs, gd, w, st, near);
-        launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st);
+        // the mark's per-fluid flags of this step go with its per-DEM flags
+        launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st, near ? w->fl_flag : nullptr);
         launch_step_dem<T>(p, D, st);
     }
     return check_launch("coupled_step");
