@@ -601,7 +601,67 @@ static void launch_step_dem(const sphdem_params_t *p, const sphdem_dem_t *D, hip
                        (T)p->dt);
 }
 
+// A second stream per device for the DEM contact work of a fused step, with
+// the events that fork it from and join it to the caller's stream.  Created
+// on the first fused step that is not under graph capture (capture then
+// records the fork and join); until they exist the step runs in one stream.
+// Default priority, non-blocking, events without timing.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+static SideStream *side_stream(hipStream_t caller) {
+    static SideStream per_device[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    SideStream &r = per_device[dev];
+    if (r.s && r.fork && r.join) return &r;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(caller, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+        return nullptr;
+    if (!r.s && hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking) != hipSuccess) r.s = nullptr;
+    if (!r.fork && hipEventCreateWithFlags(&r.fork, hipEventDisableTiming) != hipSuccess)
+        r.fork = nullptr;
+    if (!r.join && hipEventCreateWithFlags(&r.join, hipEventDisableTiming) != hipSuccess)
+        r.join = nullptr;
+    return (r.s && r.fork && r.join) ? &r : nullptr;
+}
+
+// The DEM-DEM and DEM-wall contact work of a fused step (counts, scans,
+// contact laws, CSR copies): it reads only DEM state, the DEM grids and the
+// old CSR, and writes the new CSR, D.force/torque and the flags.
+template <typename T>
+static int dem_contacts(const sphdem_params_t *p, const sphdem_dem_t *D, const sphdem_dembnd_t *W,
+                        const sphdem_grid_t *gd, const sphdem_grid_t *gw,
+                        const sphdem_work_t *w, hipStream_t st) {
+    launch_count<T, false>(D, gd, dem_inst(gd), D->radius, p->radius_max, st);
+    int rc;
+    {
+        ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
+        rc = exclusive_scan(D->contact_dem_offset_new, D->contact_dem_offset_new, D->n + 1,
+                            w->scan_tmp, st);
+    }
+    if (rc) return rc;
+    launch_contact<T, false>(D, gd, dem_inst(gd), D->radius, D->vel, D->omega, p->radius_max,
+                             p->g, p->dt, w->flags, st);
+    launch_copy_csr<T>(D, false, st);
+    launch_count<T, true>(D, gw, dem_inst(gd), W->radius, p->boundary_radius_max, st);
+    {
+        ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
+        rc = exclusive_scan(D->contact_bnd_offset_new, D->contact_bnd_offset_new, D->n + 1,
+                            w->scan_tmp, st);
+    }
+    if (rc) return rc;
+    launch_contact<T, true>(D, gw, dem_inst(gd), W->radius, W->vel, W->omega,
+                            p->boundary_radius_max, 0.0, p->dt, w->flags, st);
+    launch_copy_csr<T>(D, true, st);
+    return SPHDEM_OK;
+}
+
 // Simulation.py:102-221; see sphdem_coupled_step in sphdem.h for the stages.
+// The fused step (stage -1) runs the DEM contact work on the side stream,
+// beside the SPH passes and the porosity (which touch none of its arrays),
+// and joins it before the DEM interpolation adds the coupling forces.
 template <typename T>
 static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
                         const sphdem_dem_t *D, const sphdem_dembnd_t *W, const sphdem_grid_t *gs,
@@ -610,7 +670,36 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
                         hipStream_t st) {
     const bool all = stage < 0;
     int rc;
-    if (all || stage == 0) {
+    if (all) {
+        SideStream *side = side_stream(st);
+        hipStream_t ds = side ? side->s : st;
+        if (side) {
+            hipEventRecord(side->fork, st);
+            hipStreamWaitEvent(ds, side->fork, 0);
+        }
+        rc = dem_contacts<T>(p, D, W, gd, gw, w, ds);
+        if (side) hipEventRecord(side->join, ds);
+        if (rc) {
+            if (side) hipStreamWaitEvent(st, side->join, 0);
+            return rc;
+        }
+        rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
+        if (!rc) {
+            const double csup = 2.0 * p->h_max;
+            int *near = dem_near_flags(p, D, gd, w, csup);
+            rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near);
+            if (side) hipStreamWaitEvent(st, side->join, 0);
+            if (rc) return rc;
+            launch_interaction_dem<T>(p, D, gs, gd, w, st, near);
+            launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st,
+                                              near ? w->fl_flag : nullptr);
+            launch_step_dem<T>(p, D, st);
+            return check_launch("coupled_step");
+        }
+        if (side) hipStreamWaitEvent(st, side->join, 0);
+        return rc;
+    }
+    if (stage == 0) {
         rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
         if (rc) return rc;
         launch_count<T, false>(D, gd, dem_inst(gd), D->radius, p->radius_max, st);
@@ -621,10 +710,9 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         }
         if (rc) return rc;
     }
-    if (all || stage == 1) {
+    if (stage == 1) {
         launch_contact<T, false>(D, gd, dem_inst(gd), D->radius, D->vel, D->omega,
                                  p->radius_max, p->g, p->dt, w->flags, st);
-        if (all) launch_copy_csr<T>(D, false, st);
         launch_count<T, true>(D, gw, dem_inst(gd), W->radius, p->boundary_radius_max, st);
         {
             ProfScope prof(SPHDEM_PASS_DEM_CSR, st);
@@ -633,10 +721,9 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         }
         if (rc) return rc;
     }
-    if (all || stage == 2) {
+    if (stage == 2) {
         launch_contact<T, true>(D, gw, dem_inst(gd), W->radius, W->vel, W->omega,
                                 p->boundary_radius_max, 0.0, p->dt, w->flags, st);
-        if (all) launch_copy_csr<T>(D, true, st);
         const double csup = 2.0 * p->h_max;
         int *near = dem_near_flags(p, D, gd, w, csup);
         rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near);
