@@ -176,6 +176,9 @@ def main(argv=None) -> int:
                    "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "surrogate": sur,
         "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
+        # these run on a second stream beside the SPH passes and the porosity:
+        # their event intervals include the work they overlap
+        "passes_overlapped": ["dem_csr", "dem_contact", "interaction_dem"],
         "diagnostics_fp64": diag,
     }
     print(json.dumps(line), flush=True)

@@ -36,7 +36,7 @@ template <typename T>
 int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
                           const sphdem_grid_t *gs, const sphdem_grid_t *gd,
                           const sphdem_work_t *w, double csup, double hu, int *dem_near,
-                          hipStream_t st);
+                          hipStream_t st, hipEvent_t marked);
 
 // ---------------------------------------------------------------- contact law
 // kernel/KERNEL_DEM_force.py:25-74 DEM_contact: linear spring-dashpot with
@@ -547,11 +547,12 @@ static void launch_porosity(const sphdem_params_t *p, const sphdem_sph_t *P,
 template <typename T>
 static int porosity(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
                     const sphdem_grid_t *gs, const sphdem_grid_t *gd, const sphdem_work_t *w,
-                    double csup, double hu, hipStream_t st, int *dem_near = nullptr) {
+                    double csup, double hu, hipStream_t st, int *dem_near = nullptr,
+                    hipEvent_t marked = nullptr) {
     if (hu > 0.0) {
         SPHDEM_REQUIRE(w->fl_flag && w->fl_list && w->fl_count && w->fl_scan,
                        "porosity work buffers are NULL");
-        return launch_porosity_tiled<T>(p, P, D, gs, gd, w, csup, hu, dem_near, st);
+        return launch_porosity_tiled<T>(p, P, D, gs, gd, w, csup, hu, dem_near, st, marked);
     }
     launch_porosity<T>(p, P, D, gs, gd, w, csup, hu, st);
     return check_launch("porosity");
@@ -608,14 +609,14 @@ static void launch_step_dem(const sphdem_params_t *p, const sphdem_dem_t *D, hip
 // Default priority, non-blocking, events without timing.
 struct SideStream {
     hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, marked = nullptr;
 };
 static SideStream *side_stream(hipStream_t caller) {
     static SideStream per_device[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     SideStream &r = per_device[dev];
-    if (r.s && r.fork && r.join) return &r;
+    if (r.s && r.fork && r.join && r.marked) return &r;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(caller, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return nullptr;
@@ -624,7 +625,9 @@ static SideStream *side_stream(hipStream_t caller) {
         r.fork = nullptr;
     if (!r.join && hipEventCreateWithFlags(&r.join, hipEventDisableTiming) != hipSuccess)
         r.join = nullptr;
-    return (r.s && r.fork && r.join) ? &r : nullptr;
+    if (!r.marked && hipEventCreateWithFlags(&r.marked, hipEventDisableTiming) != hipSuccess)
+        r.marked = nullptr;
+    return (r.s && r.fork && r.join && r.marked) ? &r : nullptr;
 }
 
 // The DEM-DEM and DEM-wall contact work of a fused step (counts, scans,
@@ -673,31 +676,44 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
     if (all) {
         SideStream *side = side_stream(st);
         hipStream_t ds = side ? side->s : st;
+        auto join = [&]() {
+            if (side) {
+                hipEventRecord(side->join, ds);
+                hipStreamWaitEvent(st, side->join, 0);
+            }
+        };
         if (side) {
             hipEventRecord(side->fork, st);
             hipStreamWaitEvent(ds, side->fork, 0);
         }
         rc = dem_contacts<T>(p, D, W, gd, gw, w, ds);
-        if (side) hipEventRecord(side->join, ds);
+        if (!rc) rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
         if (rc) {
-            if (side) hipStreamWaitEvent(st, side->join, 0);
+            join();
             return rc;
         }
-        rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
-        if (!rc) {
-            const double csup = 2.0 * p->h_max;
-            int *near = dem_near_flags(p, D, gd, w, csup);
-            rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near);
-            if (side) hipStreamWaitEvent(st, side->join, 0);
-            if (rc) return rc;
-            launch_interaction_dem<T>(p, D, gs, gd, w, st, near);
-            launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st,
-                                              near ? w->fl_flag : nullptr);
-            launch_step_dem<T>(p, D, st);
-            return check_launch("coupled_step");
+        const double csup = 2.0 * p->h_max;
+        int *near = dem_near_flags(p, D, gd, w, csup);
+        // with the tiled porosity the DEM interpolation needs only the mark's
+        // flags: it runs on the side stream beside the porosity stream kernel
+        const bool beside = side && p->h_uniform > 0.0 && gs->n > 0;
+        rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near,
+                         beside ? side->marked : nullptr);
+        if (rc) {
+            join();
+            return rc;
         }
-        if (side) hipStreamWaitEvent(st, side->join, 0);
-        return rc;
+        if (beside) {
+            hipStreamWaitEvent(ds, side->marked, 0);
+            launch_interaction_dem<T>(p, D, gs, gd, w, ds, near);
+            join();
+        } else {
+            join();
+            launch_interaction_dem<T>(p, D, gs, gd, w, st, near);
+        }
+        launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st, near ? w->fl_flag : nullptr);
+        launch_step_dem<T>(p, D, st);
+        return check_launch("coupled_step");
     }
     if (stage == 0) {
         rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);

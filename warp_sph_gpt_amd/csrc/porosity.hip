@@ -508,7 +508,7 @@ template <typename T>
 int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_dem_t *D,
                           const sphdem_grid_t *gs, const sphdem_grid_t *gd,
                           const sphdem_work_t *w, double csup, double hu, int *dem_near,
-                          hipStream_t st) {
+                          hipStream_t st, hipEvent_t marked) {
     const int n = gs->n;
     if (n == 0) return SPHDEM_OK;
     ProfScope prof(SPHDEM_PASS_POROSITY, st);
@@ -518,6 +518,8 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
     hipLaunchKernelGGL(k_por_mark<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
                        view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), cs, w->fl_flag, dem_near,
                        rm);
+    // the flags are final here: work that needs only them may start
+    if (marked) hipEventRecord(marked, st);
     const T thr2 = sqrt_threshold<T>(cs);
     if (p->fast_math)
         hipLaunchKernelGGL((k_por_stream<T, true>), dim3(blocks_for(n, bt)), dim3(bt), 0, st,
@@ -532,10 +534,10 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
 template int launch_porosity_tiled<float>(const sphdem_params_t *, const sphdem_sph_t *,
                                           const sphdem_dem_t *, const sphdem_grid_t *,
                                           const sphdem_grid_t *, const sphdem_work_t *, double,
-                                          double, int *, hipStream_t);
+                                          double, int *, hipStream_t, hipEvent_t);
 template int launch_porosity_tiled<double>(const sphdem_params_t *, const sphdem_sph_t *,
                                            const sphdem_dem_t *, const sphdem_grid_t *,
                                            const sphdem_grid_t *, const sphdem_work_t *, double,
-                                           double, int *, hipStream_t);
+                                           double, int *, hipStream_t, hipEvent_t);
 
 }  // namespace sphdem
