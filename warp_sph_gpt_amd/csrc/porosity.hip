@@ -215,69 +215,120 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // D = |m|^2 + |q'|^2 - 2 m.q', the form of stream_range_rel.  The kernel
 // term stays on the VALU: each lane adds its 16 (subject, candidate) terms
 // into acc[T][t] (subject 16T + 4(l/16) + t, candidates l%16 mod 16), and the
-// 16 partial sums of a subject are added across lanes once per group.
-__device__ __forceinline__ void stream_range_mfma(float (&acc)[4][4], const float (&a)[4],
-                                                  const f32x4 (&c)[4], int a0, int b0,
-                                                  const V4<float> *pos, const float *Vs,
-                                                  V4<float> *buf, float *vbuf, int lane,
-                                                  const V3<float> &gc, float k1, float k2) {
+// 16 partial sums of a subject are added across lanes once per group.  Here:
+// nsub staged 16-candidate blocks, buf/vbuf holding candidates relative to the
+// group centre as (x', y', z', |q'|^2) and V.
+__device__ __forceinline__ void mfma_blocks(float (&acc)[4][4], const float (&a)[4],
+                                            const f32x4 (&c)[4], const V4<float> *buf,
+                                            const float *vbuf, int nsub, int lane, float k1,
+                                            float k2) {
+    const int col = lane & 15, comp = lane >> 4;
+    for (int sc = 0; sc < nsub; ++sc) {
+        // component comp of candidate 16 sc + col, read at its address (a
+        // four-way select here compiles to a per-lane branch tree)
+        const float bv = reinterpret_cast<const float *>(buf + 16 * sc + col)[comp];
+        const float vj = vbuf[16 * sc + col];
+        // V (1 + 2q) = fma(2V/h, r, V): the lane's candidate V folded in once
+        const float kv = k2 * vj;
+        // all four products in flight before the first result is read; rows
+        // of subjects outside the group sit 1e15 away and add 0
+        f32x4 d[4];
+#pragma unroll
+        for (int T = 0; T < 4; ++T)
+            d[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T], bv, c[T], 0, 0, 0);
+#pragma unroll
+        for (int T = 0; T < 4; ++T) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float r = fast_sqrt(fabsf(d[T][t]));
+                const float u = clamp01(fmaf(k1, r, 1.0f));
+                const float pqv = fmaf(kv, r, vj);
+                float w = u * u;
+                w = w * w;
+                acc[T][t] = fmaf(w, pqv, acc[T][t]);
+            }
+        }
+    }
+}
+
+// FAST, FP32, culled: a candidate farther than the support from the box of
+// the group's subjects adds 0 to every subject's sum, so it is dropped as it
+// is loaded; the others queue in the wave's LDS slot (128 entries, `fill`
+// of them pending, wave-uniform) and go through the matrix products 64 at a
+// time.  The test uses r2 = support^2 (1 + 1e-4) against a box distance
+// rounded to ~1e-7 m, so no candidate within the support is dropped.
+__device__ __forceinline__ void queue_range_mfma(float (&acc)[4][4], const float (&a)[4],
+                                                 const f32x4 (&c)[4], int &fill, int a0, int b0,
+                                                 const V4<float> *pos, const float *Vs,
+                                                 V4<float> *buf, float *vbuf, int lane,
+                                                 const V3<float> &gc, const V3<float> &bl,
+                                                 const V3<float> &bh, float r2, float k1,
+                                                 float k2) {
     if (b0 <= a0) return;
-    const float far = 1.0e15f;
     auto load = [&](int s, float &v) {
-        V4<float> q = {far, far, far, 0.0f};
+        V4<float> q = {0.0f, 0.0f, 0.0f, 0.0f};
         v = 0.0f;
         if (s < b0) {
             q = pos[s];
             v = Vs[s];
         }
-        q.x -= gc.x;
-        q.y -= gc.y;
-        q.z -= gc.z;
-        q.w = fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x));
         return q;
     };
-    const int col = lane & 15, comp = lane >> 4;
     float vcur;
     V4<float> cur = load(a0 + lane, vcur);
     for (int s0 = a0; s0 < b0; s0 += 64) {
-        buf[lane] = cur;
-        vbuf[lane] = vcur;
-        __builtin_amdgcn_wave_barrier();
         float vnxt;
         const V4<float> nxt = load(s0 + 64 + lane, vnxt);
-        const int nsub = uniform_i((min(64, b0 - s0) + 15) >> 4);
-        for (int sc = 0; sc < nsub; ++sc) {
-            // component comp of candidate 16 sc + col, read at its address (a
-            // four-way select here compiles to a per-lane branch tree)
-            const float bv = reinterpret_cast<const float *>(buf + 16 * sc + col)[comp];
-            const float vj = vbuf[16 * sc + col];
-            // V (1 + 2q) = fma(2V/h, r, V): the lane's candidate V folded in once
-            const float kv = k2 * vj;
-            auto terms = [&](int T, const f32x4 &d) {
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const float r = fast_sqrt(fabsf(d[t]));
-                    const float u = clamp01(fmaf(k1, r, 1.0f));
-                    const float pqv = fmaf(kv, r, vj);
-                    float w = u * u;
-                    w = w * w;
-                    acc[T][t] = fmaf(w, pqv, acc[T][t]);
-                }
-            };
-            // all four products in flight before the first result is read;
-            // rows of subjects outside the group sit 1e15 away and add 0
-            // (one code path beats skipping the empty tiles)
-            f32x4 d[4];
-#pragma unroll
-            for (int T = 0; T < 4; ++T)
-                d[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T], bv, c[T], 0, 0, 0);
-#pragma unroll
-            for (int T = 0; T < 4; ++T) terms(T, d[T]);
+        const float ex = fmaxf(fmaxf(bl.x - cur.x, cur.x - bh.x), 0.0f);
+        const float ey = fmaxf(fmaxf(bl.y - cur.y, cur.y - bh.y), 0.0f);
+        const float ez = fmaxf(fmaxf(bl.z - cur.z, cur.z - bh.z), 0.0f);
+        const bool keep = s0 + lane < b0 && fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= r2;
+        const unsigned long long mask = __ballot(keep);
+        if (keep) {
+            const int at = fill + __popcll(mask & ((1ull << lane) - 1ull));
+            V4<float> q;
+            q.x = cur.x - gc.x;
+            q.y = cur.y - gc.y;
+            q.z = cur.z - gc.z;
+            q.w = fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x));
+            buf[at] = q;
+            vbuf[at] = vcur;
         }
-        __builtin_amdgcn_wave_barrier();
+        fill = uniform_i(fill + __popcll(mask));
+        if (fill >= 64) {
+            __builtin_amdgcn_wave_barrier();
+            mfma_blocks(acc, a, c, buf, vbuf, 4, lane, k1, k2);
+            __builtin_amdgcn_wave_barrier();
+            // entries 64 .. fill-1 move down to 0 .. fill-65 (disjoint ranges)
+            if (lane < fill - 64) {
+                buf[lane] = buf[64 + lane];
+                vbuf[lane] = vbuf[64 + lane];
+            }
+            fill -= 64;
+            __builtin_amdgcn_wave_barrier();
+        }
         cur = nxt;
         vcur = vnxt;
     }
+}
+
+// The queue's last entries, padded to a multiple of 16 with candidates
+// 1e15 away (V = 0), which add 0.
+__device__ __forceinline__ void flush_queue_mfma(float (&acc)[4][4], const float (&a)[4],
+                                                 const f32x4 (&c)[4], int &fill,
+                                                 V4<float> *buf, float *vbuf, int lane,
+                                                 float k1, float k2) {
+    if (fill == 0) return;
+    const int nsub = (fill + 15) >> 4;
+    if (lane >= fill && lane < 16 * nsub) {
+        const float far = 1.0e15f;
+        buf[lane] = V4<float>{far, far, far, 3.0e30f};
+        vbuf[lane] = 0.0f;
+    }
+    __builtin_amdgcn_wave_barrier();
+    mfma_blocks(acc, a, c, buf, vbuf, nsub, lane, k1, k2);
+    __builtin_amdgcn_wave_barrier();
+    fill = 0;
 }
 
 // 1. near-DEM flag; far subjects get their porosity now.
@@ -338,13 +389,14 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
     const T m2 = dot(m, m);
     const T k1 = (T)-0.5 * inv_h, k2 = (T)2.0 * inv_h;
     const int dxy = gs.dx * gs.dy;
-    // FP32 FAST: matrix-core distances (stream_range_mfma); set up the
+    // FP32 FAST: matrix-core distances (mfma_blocks, fed by queue_range_mfma); set up the
     // group's A rows and C values, accumulate, reduce at the end
     constexpr bool kMfma = FAST && std::is_same<T, float>::value;
     const int col = lane & 15, comp = lane >> 4;
     float ma[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     f32x4 mc[4];
     float acc[4][4] = {};
+    int fill = 0;
     if constexpr (kMfma) {
 #pragma unroll
         for (int Tt = 0; Tt < 4; ++Tt) {
@@ -376,12 +428,14 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
             a2 = uniform_i(a2); b2 = uniform_i(b2);
             if constexpr (kMfma) {
                 const V3<float> gcf = mk3((float)gc.x, (float)gc.y, (float)gc.z);
-                stream_range_mfma(acc, ma, mc, a1, b1, (const V4<float> *)gs.s,
-                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
-                                  (float)k1, (float)k2);
-                stream_range_mfma(acc, ma, mc, a2, b2, (const V4<float> *)gs.s,
-                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
-                                  (float)k1, (float)k2);
+                const V3<float> blf = mk3((float)bxl, (float)byl, (float)bzl);
+                const V3<float> bhf = mk3((float)bxh, (float)byh, (float)bzh);
+                queue_range_mfma(acc, ma, mc, fill, a1, b1, (const V4<float> *)gs.s,
+                                 (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
+                                 blf, bhf, (float)r2c, (float)k1, (float)k2);
+                queue_range_mfma(acc, ma, mc, fill, a2, b2, (const V4<float> *)gs.s,
+                                 (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
+                                 blf, bhf, (float)r2c, (float)k1, (float)k2);
             } else if (FAST) {
                 stream_range_rel(flt, mx2, m2, a1, b1, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
                 stream_range_rel(flt, mx2, m2, a2, b2, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
@@ -392,6 +446,8 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
         }
     }
     if constexpr (kMfma) {
+        flush_queue_mfma(acc, ma, mc, fill, (V4<float> *)wbuf, (float *)vbuf, lane, (float)k1,
+                         (float)k2);
         // subject 16T + 4(l/16) + t: add its 16 column partial sums, lanes
         // l%16 = 0..15 of each quarter (fixed xor tree), then hand the total
         // to the subject's own lane through the wave's LDS slot
@@ -431,12 +487,14 @@ template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
                                                     GridV<T> gd, const T *Vs, const int *flag,
                                                     T csup, T hu, T thr2, int xt, int remap) {
-    __shared__ V4<T> stage[256];
-    __shared__ T vstage[256];
+    // per wave: 64 staged candidates, 128 for the FP32 matrix-core queue
+    constexpr int kSlot = (FAST && std::is_same<T, float>::value) ? 128 : 64;
+    __shared__ V4<T> stage[4 * kSlot];
+    __shared__ T vstage[4 * kSlot];
     const int k = subject_id(remap);
     const int lane = threadIdx.x & 63;
-    V4<T> *wbuf = stage + (threadIdx.x & ~63);
-    T *vbuf = vstage + (threadIdx.x & ~63);
+    V4<T> *wbuf = stage + (threadIdx.x >> 6) * kSlot;
+    T *vbuf = vstage + (threadIdx.x >> 6) * kSlot;
     const bool active = k < gs.n && flag[k] != 0;
     const unsigned long long live = __ballot(active);
     if (live == 0ull) return;
