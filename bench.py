@@ -77,6 +77,10 @@ def main(argv=None) -> int:
                                                        "profiles", "porosity_c2_traffic.json"),
                     help="JSON file with measured HBM bytes per launch of the dominant pass "
                          "(from rocprofv3 FETCH_SIZE/WRITE_SIZE, scaled by a copy of known size)")
+    ap.add_argument("--counters", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                        "profiles", "porosity_c2_valu.json"),
+                    help="JSON file with rocprofv3 instruction counters of the dominant pass "
+                         "(matrix products per launch, VALU issue fraction)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy a buffer of this many MiB 5 times (counter calibration)")
     args = ap.parse_args(argv)
@@ -153,6 +157,18 @@ def main(argv=None) -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                 "kernel": dom, "avg_launch_ms": round(avg_s * 1e3, 4),
                 "bytes_per_launch": int(bytes_launch)}
+    # pair rate of the dominant pass: pairs per launch from its counter
+    # profile (16 x 16 pairs per matrix product) over this run's launch time
+    counters = None
+    if args.counters and os.path.exists(args.counters) and avg_s > 0:
+        with open(args.counters) as f:
+            c = json.load(f)
+        if c.get("kernel") == dom and c.get("workload") == wl.name and c.get("math") == args.math:
+            counters = {"pairs_evaluated_per_launch": c["pairs_evaluated_per_launch"],
+                        "pair_rate": round(c["pairs_evaluated_per_launch"] / avg_s, 1),
+                        "valu_issue_frac": c["valu_issue_frac"],
+                        "source": os.path.relpath(args.counters,
+                                                  os.path.dirname(os.path.abspath(__file__)))}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -175,6 +191,7 @@ def main(argv=None) -> int:
                    "graph": bool(args.graph), "math": args.math, "tuning": tuned,
                    "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "surrogate": sur,
+        "dominant_pass_counters": counters,
         "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
         # these run on a second stream beside the SPH passes and the porosity:
         # their event intervals include the work they overlap
