@@ -279,6 +279,27 @@ def test_coupling_kernels(uniform_h, math_mode):
     S.compare(["pos", "vel"])
 
 
+@pytest.mark.parametrize("grid_slice", [12, 10, 8])
+def test_porosity_on_folded_grids(grid_slice, math_mode):
+    """The coupling-scale porosity with the tank folded onto a few buckets per
+    axis.  At 12 the wave groups' boxes (query radius ~3 buckets) fit within
+    dims - 2, and bucket rows wrap around the fold (two slot ranges per row).
+    At 10 some groups fit and some do not, and at 8 none do: the sub-group and
+    per-subject paths run."""
+    S = Scene(grid_slice=grid_slice)
+    s = S.solv
+    (P, B, D, W), (gP, gB, gD, gW) = S.o, S.g
+    rmax, rbmax, hmax, hu = S.scales
+    O.Kernel_prep_sphdem(P, B, D, *S.og[:3], 2 * hmax, hu, s.support, s.h, s.dem_porosity_min,
+                         s.dem_porosity_max)
+    K.Kernel_prep_sphdem(gP, gB, gD, *S.gg[:3], 2 * hmax, hu, s.support, s.h,
+                         s.dem_porosity_min, s.dem_porosity_max)
+    S.compare(["porosity", "pgf"])
+    assert P.porosity.min() < 0.99
+    if math_mode == "exact":
+        np.testing.assert_array_equal(np_(gP.porosity), P.porosity)
+
+
 def test_oracle_prep_skip_changes_nothing():
     """The oracle's skip of the coupling-scale fluid sum where no DEM is
     within 2h_b leaves porosity and -grad p bit-identical."""
