@@ -218,7 +218,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // 16 partial sums of a subject are added across lanes once per group.  Here:
 // nsub staged 16-candidate blocks, buf/vbuf holding candidates relative to the
 // group centre as (x', y', z', |q'|^2) and V.
-__device__ __forceinline__ void mfma_blocks(float (&acc)[4][4], const float (&a)[4],
+__device__ __forceinline__ void mfma_blocks(f32x4 (&acc)[4], const float (&a)[4],
                                             const f32x4 (&c)[4], const V4<float> *buf,
                                             const float *vbuf, int nsub, int lane, float k1,
                                             float k2) {
@@ -257,7 +257,7 @@ __device__ __forceinline__ void mfma_blocks(float (&acc)[4][4], const float (&a)
 // of them pending, wave-uniform) and go through the matrix products 64 at a
 // time.  The test uses r2 = support^2 (1 + 1e-4) against a box distance
 // rounded to ~1e-7 m, so no candidate within the support is dropped.
-__device__ __forceinline__ void queue_range_mfma(float (&acc)[4][4], const float (&a)[4],
+__device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&a)[4],
                                                  const f32x4 (&c)[4], int &fill, int a0, int b0,
                                                  const V4<float> *pos, const float *Vs,
                                                  V4<float> *buf, float *vbuf, int lane,
@@ -314,7 +314,7 @@ __device__ __forceinline__ void queue_range_mfma(float (&acc)[4][4], const float
 
 // The queue's last entries, padded to a multiple of 16 with candidates
 // 1e15 away (V = 0), which add 0.
-__device__ __forceinline__ void flush_queue_mfma(float (&acc)[4][4], const float (&a)[4],
+__device__ __forceinline__ void flush_queue_mfma(f32x4 (&acc)[4], const float (&a)[4],
                                                  const f32x4 (&c)[4], int &fill,
                                                  V4<float> *buf, float *vbuf, int lane,
                                                  float k1, float k2) {
@@ -395,7 +395,7 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
     const int col = lane & 15, comp = lane >> 4;
     float ma[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     f32x4 mc[4];
-    float acc[4][4] = {};
+    f32x4 acc[4] = {};
     int fill = 0;
     if constexpr (kMfma) {
 #pragma unroll
