@@ -255,41 +255,41 @@ __device__ __forceinline__ void mfma_blocks(f32x4 (&acc)[4], const float (&a)[4]
 // the group's subjects adds 0 to every subject's sum, so it is dropped as it
 // is loaded; the others queue in the wave's LDS slot (128 entries, `fill`
 // of them pending, wave-uniform) and go through the matrix products 64 at a
-// time.  The test uses r2 = support^2 (1 + 1e-4) against a box distance
-// rounded to ~1e-7 m, so no candidate within the support is dropped.
+// time.  The box is centred on gc with half-extents he, so its distance is
+// measured on the relative coordinates the queue stores.  The test uses
+// r2 = support^2 (1 + 1e-4) against a distance rounded to ~1e-7 m, so no
+// candidate within the support is dropped.
 __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&a)[4],
                                                  const f32x4 (&c)[4], int &fill, int a0, int b0,
                                                  const V4<float> *pos, const float *Vs,
                                                  V4<float> *buf, float *vbuf, int lane,
-                                                 const V3<float> &gc, const V3<float> &bl,
-                                                 const V3<float> &bh, float r2, float k1,
-                                                 float k2) {
+                                                 const V3<float> &gc, const V3<float> &he,
+                                                 float r2, float k1, float k2) {
     if (b0 <= a0) return;
+    // slots past the range read its last slot again and are never kept
     auto load = [&](int s, float &v) {
-        V4<float> q = {0.0f, 0.0f, 0.0f, 0.0f};
-        v = 0.0f;
-        if (s < b0) {
-            q = pos[s];
-            v = Vs[s];
-        }
-        return q;
+        const int t = min(s, b0 - 1);
+        v = Vs[t];
+        return pos[t];
     };
     float vcur;
     V4<float> cur = load(a0 + lane, vcur);
     for (int s0 = a0; s0 < b0; s0 += 64) {
         float vnxt;
         const V4<float> nxt = load(s0 + 64 + lane, vnxt);
-        const float ex = fmaxf(fmaxf(bl.x - cur.x, cur.x - bh.x), 0.0f);
-        const float ey = fmaxf(fmaxf(bl.y - cur.y, cur.y - bh.y), 0.0f);
-        const float ez = fmaxf(fmaxf(bl.z - cur.z, cur.z - bh.z), 0.0f);
+        V4<float> q;
+        q.x = cur.x - gc.x;
+        q.y = cur.y - gc.y;
+        q.z = cur.z - gc.z;
+        const float ex = fmaxf(fabsf(q.x) - he.x, 0.0f);
+        const float ey = fmaxf(fabsf(q.y) - he.y, 0.0f);
+        const float ez = fmaxf(fabsf(q.z) - he.z, 0.0f);
         const bool keep = s0 + lane < b0 && fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= r2;
         const unsigned long long mask = __ballot(keep);
         if (keep) {
-            const int at = fill + __popcll(mask & ((1ull << lane) - 1ull));
-            V4<float> q;
-            q.x = cur.x - gc.x;
-            q.y = cur.y - gc.y;
-            q.z = cur.z - gc.z;
+            const int at = fill + (int)__builtin_amdgcn_mbcnt_hi(
+                                      (unsigned)(mask >> 32),
+                                      __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
             q.w = fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x));
             buf[at] = q;
             vbuf[at] = vcur;
@@ -428,14 +428,15 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
             a2 = uniform_i(a2); b2 = uniform_i(b2);
             if constexpr (kMfma) {
                 const V3<float> gcf = mk3((float)gc.x, (float)gc.y, (float)gc.z);
-                const V3<float> blf = mk3((float)bxl, (float)byl, (float)bzl);
-                const V3<float> bhf = mk3((float)bxh, (float)byh, (float)bzh);
+                const V3<float> hef = mk3((float)((bxh - bxl) * (T)0.5),
+                                          (float)((byh - byl) * (T)0.5),
+                                          (float)((bzh - bzl) * (T)0.5));
                 queue_range_mfma(acc, ma, mc, fill, a1, b1, (const V4<float> *)gs.s,
                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
-                                 blf, bhf, (float)r2c, (float)k1, (float)k2);
+                                 hef, (float)r2c, (float)k1, (float)k2);
                 queue_range_mfma(acc, ma, mc, fill, a2, b2, (const V4<float> *)gs.s,
                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
-                                 blf, bhf, (float)r2c, (float)k1, (float)k2);
+                                 hef, (float)r2c, (float)k1, (float)k2);
             } else if (FAST) {
                 stream_range_rel(flt, mx2, m2, a1, b1, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
                 stream_range_rel(flt, mx2, m2, a2, b2, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
