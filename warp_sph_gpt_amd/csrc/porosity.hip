@@ -409,40 +409,73 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
             for (int t = 0; t < 4; ++t) mc[Tt][t] = __shfl((float)m2, 16 * Tt + 4 * comp + t, 64);
         }
     }
-    for (int z = Z0; z <= Z1; ++z) {
-        const T gz = axis_gap(bzl, bzh, z, w);
-        if (gz * gz > r2c) continue;
-        const int cz = fold(z, gs.dz);
-        for (int y = Y0; y <= Y1; ++y) {
-            const T gy = axis_gap(byl, byh, y, w);
-            const T dd = gy * gy + gz * gz;
-            if (dd > r2c) continue;
-            const T chord = msqrt(r2c - dd);
-            const int xa = uniform_i(max(X0, (int)((bxl - chord) * inv) - 1));
-            const int xb = uniform_i(min(X1, (int)((bxh + chord) * inv) + 1));
-            if (xb < xa) continue;
-            const int row = cz * dxy + fold(y, gs.dy) * gs.dx;
-            int a1, b1, a2, b2;
-            row_slots(gs.lo + gs.off, gs.hi + gs.off, row, gs.dx, xa, xb, a1, b1, a2, b2);
-            a1 = uniform_i(a1); b1 = uniform_i(b1);
-            a2 = uniform_i(a2); b2 = uniform_i(b2);
-            if constexpr (kMfma) {
-                const V3<float> gcf = mk3((float)gc.x, (float)gc.y, (float)gc.z);
-                const V3<float> hef = mk3((float)((bxh - bxl) * (T)0.5),
-                                          (float)((byh - byl) * (T)0.5),
-                                          (float)((bzh - bzl) * (T)0.5));
+    if constexpr (kMfma) {
+        // FP32 FAST: the lanes find the slot ranges of 64 bucket rows at once
+        // (row r0 + lane, z-major as the loops below), so the bucket-bound
+        // loads of a row are not waited for one row at a time; the wave then
+        // queues the rows' candidates in order
+        const V3<float> gcf = mk3((float)gc.x, (float)gc.y, (float)gc.z);
+        const V3<float> hef = mk3((float)((bxh - bxl) * (T)0.5), (float)((byh - byl) * (T)0.5),
+                                  (float)((bzh - bzl) * (T)0.5));
+        const int ny = Y1 - Y0 + 1, nrows = (Z1 - Z0 + 1) * ny;
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+            int ra1 = 0, rb1 = 0, ra2 = 0, rb2 = 0;
+            const int idx = r0 + lane;
+            if (idx < nrows) {
+                const int z = Z0 + idx / ny, y = Y0 + idx % ny;
+                const T gz = axis_gap(bzl, bzh, z, w);
+                const T gy = axis_gap(byl, byh, y, w);
+                const T dd = gy * gy + gz * gz;
+                if (dd <= r2c) {
+                    const T chord = msqrt(r2c - dd);
+                    const int xa = max(X0, (int)((bxl - chord) * inv) - 1);
+                    const int xb = min(X1, (int)((bxh + chord) * inv) + 1);
+                    const int row = fold(z, gs.dz) * dxy + fold(y, gs.dy) * gs.dx;
+                    row_slots(gs.lo + gs.off, gs.hi + gs.off, row, gs.dx, xa, xb, ra1, rb1, ra2,
+                              rb2);
+                }
+            }
+            const int nr = min(64, nrows - r0);
+            for (int r = 0; r < nr; ++r) {
+                const int a1 = __builtin_amdgcn_readlane(ra1, r);
+                const int b1 = __builtin_amdgcn_readlane(rb1, r);
+                const int a2 = __builtin_amdgcn_readlane(ra2, r);
+                const int b2 = __builtin_amdgcn_readlane(rb2, r);
                 queue_range_mfma(acc, ma, mc, fill, a1, b1, (const V4<float> *)gs.s,
                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
                                  hef, (float)r2c, (float)k1, (float)k2);
                 queue_range_mfma(acc, ma, mc, fill, a2, b2, (const V4<float> *)gs.s,
                                  (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
                                  hef, (float)r2c, (float)k1, (float)k2);
-            } else if (FAST) {
-                stream_range_rel(flt, mx2, m2, a1, b1, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
-                stream_range_rel(flt, mx2, m2, a2, b2, gs.s, Vs, wbuf, vbuf, lane, gc, k1, k2);
-            } else {
-                stream_range<T, FAST>(flt, me, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
-                stream_range<T, FAST>(flt, me, a2, b2, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+            }
+        }
+    } else {
+        for (int z = Z0; z <= Z1; ++z) {
+            const T gz = axis_gap(bzl, bzh, z, w);
+            if (gz * gz > r2c) continue;
+            const int cz = fold(z, gs.dz);
+            for (int y = Y0; y <= Y1; ++y) {
+                const T gy = axis_gap(byl, byh, y, w);
+                const T dd = gy * gy + gz * gz;
+                if (dd > r2c) continue;
+                const T chord = msqrt(r2c - dd);
+                const int xa = uniform_i(max(X0, (int)((bxl - chord) * inv) - 1));
+                const int xb = uniform_i(min(X1, (int)((bxh + chord) * inv) + 1));
+                if (xb < xa) continue;
+                const int row = cz * dxy + fold(y, gs.dy) * gs.dx;
+                int a1, b1, a2, b2;
+                row_slots(gs.lo + gs.off, gs.hi + gs.off, row, gs.dx, xa, xb, a1, b1, a2, b2);
+                a1 = uniform_i(a1); b1 = uniform_i(b1);
+                a2 = uniform_i(a2); b2 = uniform_i(b2);
+                if (FAST) {
+                    stream_range_rel(flt, mx2, m2, a1, b1, gs.s, Vs, wbuf, vbuf, lane, gc, k1,
+                                     k2);
+                    stream_range_rel(flt, mx2, m2, a2, b2, gs.s, Vs, wbuf, vbuf, lane, gc, k1,
+                                     k2);
+                } else {
+                    stream_range<T, FAST>(flt, me, a1, b1, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+                    stream_range<T, FAST>(flt, me, a2, b2, gs.s, Vs, wbuf, lane, thr, h, inv_h);
+                }
             }
         }
     }
