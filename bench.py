@@ -169,6 +169,8 @@ def main(argv=None) -> int:
                         "valu_issue_frac": c["valu_issue_frac"],
                         "source": os.path.relpath(args.counters,
                                                   os.path.dirname(os.path.abspath(__file__)))}
+            # the pass is bound by vector-ALU issue, not by HBM bytes
+            roofline["limiter"] = {"what": "valu_issue", "frac": c["valu_issue_frac"]}
 
     cpu = None
     if not args.no_cpu_baseline:
