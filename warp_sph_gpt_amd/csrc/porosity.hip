@@ -9,19 +9,23 @@
 //   2. stream: one wave per 64 consecutive fluid slots holding a flagged
 //              subject.  The wave walks the buckets of the union of its
 //              subjects' balls in the reference's query order (z, then y,
-//              then x, then sorted order inside a bucket), one candidate at a
-//              time for all 64 lanes: candidate data come through wave-uniform
-//              (scalar) loads, and each lane adds the term only if the
-//              candidate lies within 2 h_dem of its own subject.  Every
-//              candidate that passes the reference's distance test lies in
-//              the union and inside the subject's own query box, so each sum
-//              receives exactly the reference's terms in the reference's
+//              then x, then sorted order inside a bucket), 64 candidates at a
+//              time: one coalesced load per lane, staged in the wave's LDS
+//              slot and read back by every lane.  Each lane adds the term
+//              only if the candidate lies within 2 h_dem of its own subject.
+//              Every candidate that passes the reference's distance test lies
+//              in the union and inside the subject's own query box, so each
+//              sum receives exactly the reference's terms in the reference's
 //              order; buckets that cannot hold such a term are skipped.
 // Math: EXACT evaluates W with IEEE division and square root as written in
-// kernel/Kernel_KNL.py; FAST uses the hardware square root, fused
-// multiply-adds with 1/h folded in, the factored polynomial C u^4 (1+2q)
-// with C applied once per sum, and u = 1 - q/2 clamped to [0, 1] in place
-// of the support test.
+// kernel/Kernel_KNL.py, in that order (bit-exact).  FAST uses the hardware
+// square root, fused multiply-adds with 1/h folded in, the factored
+// polynomial C u^4 (1+2q) with C applied once per sum, and u = 1 - q/2
+// clamped to [0, 1] in place of the support test; in FP32 the squared
+// distances come from the matrix cores and candidates that cannot reach any
+// subject of the group are dropped before them (mfma_blocks and
+// queue_range_mfma below), which changes the rounding of the sums, not
+// their terms.
 #include "common.hpp"
 
 #include <climits>
