@@ -140,6 +140,14 @@ typedef struct {
      * density from here (the same bits), and the pass fills a NaN on first
      * use.  NULL: every wall sum is evaluated. */
     void *wl_static;
+    /* [6 * 257] or NULL: scratch of the fused step's fluid bounding box
+     * (min x, y, z and max x, y, z of the fluid grid's sorted positions,
+     * all instances; 256 partial boxes, then the box).  The wall pass skips
+     * the fluid query of a wall particle that lies farther than the support
+     * from this box along some axis: that sum has no term (+0).  Used for
+     * wall sets of 4M particles or more, where it pays.  NULL: every wall
+     * particle runs its fluid query. */
+    void *fl_box;
     int cap_f, cap_b, cap_d;
 } sphdem_work_t;
 

@@ -23,6 +23,7 @@ from helpers import COMPACT, assert_rel, generation, np_, particles, solv_for, t
 from oracle import oracle as O
 from warp_sph_gpt_amd.engine import CoupledEngine
 from warp_sph_gpt_amd.shadow import DEM_FIELDS, SPH_FIELDS, ShadowRun
+from warp_sph_gpt_amd.struct import BNDptl
 from warp_sph_gpt_amd.workloads import c0
 
 pytestmark = pytest.mark.gpu
@@ -200,3 +201,41 @@ def test_static_wall_densities_change_nothing():
     assert 0 < int(known.sum()) < cached.B.n
     cached.walls_changed()
     assert not bool(torch.isfinite(cached.ws.wl_static).any())
+
+
+def test_fluid_box_skip_changes_nothing():
+    """On wall sets of 4M particles or more the wall pass skips the fluid
+    query of a wall particle farther than the support from the fluid's
+    bounding box (that sum is +0).  The shipped dam break with 4.2M extra
+    wall particles on a lattice above the tank: two coupled steps with the
+    skip and without it leave every field equal bit for bit."""
+    wl = c0(device=DEV)
+    P, B, D, W = [s.to(DEV) for s in wl.structs]
+    n = 162
+    ax = torch.arange(n, dtype=torch.float32) * 0.02
+    gx, gy, gz = torch.meshgrid(ax - 1.0, ax - 1.0, ax[:161] + 1.5, indexing="ij")
+    extra = torch.stack([gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)], 1).to(DEV)
+    m = extra.shape[0]
+    B2 = BNDptl.allocate(B.n + m, device=DEV)
+    for name in ("pos", "vel", "rho_raw", "rho", "pres", "acc", "m"):
+        getattr(B2, name)[:B.n] = getattr(B, name)
+    B2.pos[B.n:] = extra
+    B2.rho_raw[B.n:] = B.rho_raw[0]
+    B2.rho[B.n:] = B.rho[0]
+    B2.m[B.n:] = B.m[0]
+    assert B2.n >= 1 << 22
+    skip = CoupledEngine(wl.solv, *[s.clone() for s in (P, B2, D, W)])
+    plain = CoupledEngine(wl.solv, *[s.clone() for s in (P, B2, D, W)])
+    plain.ws.fl_box = None
+    for _ in range(2):
+        skip.step()
+        plain.step()
+    torch.cuda.synchronize()
+    for name in ("pos", "vel", "acc", "rho", "pres", "flt"):
+        assert torch.equal(getattr(skip.P, name), getattr(plain.P, name)), name
+    for name in ("rho", "pres"):
+        assert torch.equal(getattr(skip.B, name), getattr(plain.B, name)), f"wall {name}"
+    for name in ("pos", "vel", "omega", "force", "drag"):
+        assert torch.equal(getattr(skip.D, name), getattr(plain.D, name)), name
+    # the far lattice has wall neighbours but no fluid term
+    assert float(skip.B.rho[B.n:].min()) > 0.0

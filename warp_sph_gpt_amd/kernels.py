@@ -29,6 +29,9 @@ class Workspace:
         self.cap_f = self.cap_b = self.cap_scan = self.cap_d = -1
         self.wl_static = None   # see static_walls()
         self.flags = torch.zeros(4, dtype=torch.int32, device=self.device)
+        # fluid bounding box scratch (sphdem_work_t.fl_box), refilled every
+        # step; None turns the wall pass's far-wall skip off
+        self.fl_box = torch.zeros(6 * 257, dtype=dtype, device=self.device)
         self.reserve(1, 1, 1)
 
     def reserve(self, nf: int, nb: int, nd: int) -> None:
@@ -78,6 +81,7 @@ class Workspace:
                      "dem_near"):
             setattr(w, name, getattr(self, name).data_ptr())
         w.wl_static = self.wl_static.data_ptr() if self.wl_static is not None else None
+        w.fl_box = self.fl_box.data_ptr() if self.fl_box is not None else None
         w.cap_f, w.cap_b, w.cap_d = self.cap_f, self.cap_b, self.cap_d
         return w
 
