@@ -170,13 +170,32 @@ template <typename T>
 __global__ void k_grid_key(const T *pos, int n, T inv, int dx, int dy, int dz, const int *inst,
                            int *cells, int *rank, int *count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int x = (int)(pos[3 * i] * inv), y = (int)(pos[3 * i + 1] * inv),
-              z = (int)(pos[3 * i + 2] * inv);
-    int key = fold(z, dz) * (dx * dy) + fold(y, dy) * dx + fold(x, dx);
-    if (inst) key += inst[i] * (dx * dy * dz);
-    cells[i] = key;
-    rank[i] = atomicAdd(&count[key], 1);
+    const bool live = i < n;
+    int key = -1;
+    if (live) {
+        const int x = (int)(pos[3 * i] * inv), y = (int)(pos[3 * i + 1] * inv),
+                  z = (int)(pos[3 * i + 2] * inv);
+        key = fold(z, dz) * (dx * dy) + fold(y, dy) * dx + fold(x, dx);
+        if (inst) key += inst[i] * (dx * dy * dz);
+        cells[i] = key;
+    }
+    // consecutive points of one bucket (runs of lanes with the same key)
+    // take their ranks with one atomic per run: the first lane of the run
+    // reserves the run's length and each lane takes base + its place in the
+    // run.  Ranks only need to be distinct within a bucket (k_grid_order
+    // puts every bucket in ascending point order afterwards).
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const unsigned long long starts = __ballot(live && (lane == 0 || prev != key));
+    const unsigned long long upto = starts & (~0ull >> (63 - lane));
+    const int head = 63 - __builtin_clzll(upto | 1ull);
+    const unsigned long long later = starts & ~(~0ull >> (63 - lane));
+    const unsigned long long lives = __ballot(live);
+    const int end = later ? __builtin_ctzll(later) : 64 - __builtin_clzll(lives);
+    int base = 0;
+    if (live && lane == head) base = atomicAdd(&count[key], end - head);
+    base = __shfl(base, head, 64);
+    if (live) rank[i] = base + (lane - head);
 }
 
 __global__ void k_grid_scatter(const int *cells, const int *rank, const int *lo, int n, int *ids) {
