@@ -211,8 +211,16 @@ __global__ void k_grid_order(const int *cells, const int *lo, const int *cnt, co
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = cells[i], a = lo[c], e = a + cnt[c];
-    int pos = a;
-    for (int k = a; k < e; ++k) pos += tmp[k] < i;
+    int pos = a, k = a;
+    // eight independent loads in flight, not one load and wait per slot
+    for (; k + 8 <= e; k += 8) {
+        int t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = tmp[k + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pos += t[u] < i;
+    }
+    for (; k < e; ++k) pos += tmp[k] < i;
     ids[pos] = i;
 }
 
