@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
 //  rho = 0 + (the same wall terms in the same order), a function of the
 //  static wall positions and masses, so it is taken from there (and stored
 //  on first use).  Valid while the wall grid, B.pos and B.m are unchanged.
-template <typename T, bool WORK>
+template <typename T, bool WORK, int U>
 __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, BndV<T> B,
                                                            GridV<T> gs, GridV<T> gb, WorkV<T> w,
                                                            T *wall_only, const T *fbox,
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, 
     T rho = (T)0;
     auto acc = [&](const GridV<T> &g) {
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
-            each_slot<4>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int, const V4<T> &q) {
+            each_slot<U>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int, const V4<T> &q) {
                 const T dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
                 const T td = dx * dx + dy * dy + dz * dz;
                 if (td < ss) rho = rho + q.w * wendland_w(msqrt(td), wk);
@@ -394,9 +394,17 @@ static void launch_wall_density(const sphdem_params_t *p, const sphdem_sph_t *P,
         hipLaunchKernelGGL(k_box_fold<T>, dim3(1), dim3(64), 0, st, box);
         fbox = box + 6 * kBoxParts;
     }
-    hipLaunchKernelGGL((k_wall_density<T, WORK>), dim3(blocks_for(gb->n, bt)), dim3(bt), 0, st,
-                       prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb), wv,
-                       wall_only, fbox, tuned_remap(SPHDEM_PASS_WALL_DENSITY));
+    // below that size the pass is set by the chains of the walls near the
+    // fluid, which are few waves: eight candidate loads in flight instead of
+    // four (1.08M walls: 0.615 -> 0.540 ms; 11.5M walls: 1.094 -> 1.113 ms)
+    if (gb->n >= kBoxMinWalls)
+        hipLaunchKernelGGL((k_wall_density<T, WORK, 4>), dim3(blocks_for(gb->n, bt)), dim3(bt), 0,
+                           st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb), wv,
+                           wall_only, fbox, tuned_remap(SPHDEM_PASS_WALL_DENSITY));
+    else
+        hipLaunchKernelGGL((k_wall_density<T, WORK, 8>), dim3(blocks_for(gb->n, bt)), dim3(bt), 0,
+                           st, prm<T>(p), view<T>(P), view<T>(B), view<T>(gs), view<T>(gb), wv,
+                           wall_only, fbox, tuned_remap(SPHDEM_PASS_WALL_DENSITY));
 }
 
 template <typename T, bool ACC, bool PGF, bool PUBLISH>
