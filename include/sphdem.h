@@ -248,6 +248,14 @@ int sphdem_sph_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
                     const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step,
                     int integrate, const sphdem_work_t *w, void *stream);
 
+/* Per-device set-up of the fused coupled step (no reference counterpart;
+ * Warp creates its streams on wp.init()): the side stream and events the
+ * DEM contact work runs on, for the current device.  Call once per device
+ * before capturing a coupled step into a hipGraph, outside capture; a first
+ * step outside capture does the same.  Until then a captured step records a
+ * single-stream step (the same results, without the overlap). */
+int sphdem_device_init(void *stream);
+
 /* source/Simulation.py:102-221 SPHDEM_OneStep, split at the two points where
  * the reference reads the CSR totals back to the host:
  *   stage 0: SPH fields and forces, DEM-DEM counts + scan (E_dem = offset[n])
@@ -321,7 +329,7 @@ int sphdem_reduce_f64(int precision, const void *sph_pos, const void *sph_vel,
 
 /* ---- learned surrogate (no reference counterpart) ---------------------- */
 /* A two-round message-passing network over the SPH neighbourhoods of the
- * fluid and wall grids (width 64, ReLU), BF16 operands on MFMA with FP32
+ * fluid and wall grids (width 64, SiLU), BF16 operands on MFMA with FP32
  * accumulation; it stands in for ten WCSPH substeps.  All node arrays are in
  * the fluid grid's sorted order, row-major [N x 64] (bf16 arrays as uint16
  * bit patterns).  Weights are FP32 [K x 64]; sphdem_mpnn_pack writes their
@@ -342,7 +350,7 @@ int sphdem_mpnn_dense_bwd_x(const float *dY, const float *Ymask, int rows, const
 int sphdem_mpnn_dense_bwd_w(const float *dY, const float *Ymask, const void *X0, int K0,
                             const void *X1, int K1, int rows, float *gW, float *gb,
                             void *stream);
-/* s_i = (1/64) sum_j relu(relu(P_i + Q_j + e_ij C + c1) W2 + c2) over the
+/* s_i = (1/64) sum_j silu(silu(P_i + Q_j + e_ij C + c1) W2 + c2) over the
  * fluid and wall neighbours j of fluid node i (1e-12 < r^2 < support^2);
  * Q_j = Qw for walls.  fvel: [N x 4] sorted fluid velocity (from
  * sphdem_mpnn_features); wvel: [nb x 4] sorted wall velocity or NULL. */

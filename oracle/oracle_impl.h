@@ -13,11 +13,19 @@
  * How each part is pinned (tests/ hold the checks):
  *   wendland_w / wendland_dw  closed form; unit integral over the 2h ball;
  *                             finite-difference slope; end values.
- *   grid_build / query        brute-force O(N^2) neighbour sets on small
- *                             clouds; stable-permutation and partition
- *                             invariants.  The bucket key and traversal order
- *                             restate Warp's hashgrid (third-party, absent):
- *                             parity of keys/order against Warp is UNPINNED.
+ *   grid_build / query        a hand-derived table of bucket keys, stable
+ *                             order and bounds (truncation toward zero, the
+ *                             2^20 origin on dims that do not divide it, the
+ *                             fold and the clamp; tests/pins.py); brute-force
+ *                             O(N^2) neighbour sets on small clouds;
+ *                             stable-permutation and partition invariants.
+ *                             The convention restates Warp's hashgrid
+ *                             (third-party, absent): that Warp itself keys
+ *                             and orders this way stays unverified.
+ *   pres (Tait EOS)           the closed form evaluated exactly with
+ *                             rationals over [0.98, 1.05] rho0 (gamma 7 and
+ *                             1), rho = rho0 and the clamped side, and hand
+ *                             values (tests/test_oracle_pins.py).
  *   density / shepard / force brute-force FP64 sums on small scenes; lattice
  *                             and hydrostatic invariants; gravity-at-rest.
  *   DEM contact / CSR         closed-form contact tests and CSR sequences of

@@ -52,3 +52,27 @@ def test_ctypes_structures_follow_the_header(cname, pyname):
     fields = struct_fields(cname)
     got = [f[0] for f in getattr(_lib, pyname)._fields_]
     assert got == fields, (cname, fields, got)
+
+
+INTEGRATION = (ROOT / "INTEGRATION.md").read_text()
+
+
+def integration_struct(name):
+    """Run the ctypes class `name` of INTEGRATION.md's reference-side binding
+    (its definition only) and return it."""
+    code = re.search(r"```python\n(.*?)```", INTEGRATION, re.S).group(1)
+    start = code.index(f"class {name}(ctypes.Structure)")
+    end = code.index("\n\n", start)
+    ns = {"ctypes": ctypes}
+    exec(code[start:end], ns)
+    return ns[name]
+
+
+def test_integration_grid_binding_matches_the_header():
+    """The documented ctypes Grid has every field of sphdem_grid_t, in order,
+    with the library's own binding's layout (size and offsets)."""
+    Grid = integration_struct("Grid")
+    assert [f[0] for f in Grid._fields_] == struct_fields("sphdem_grid_t")
+    assert ctypes.sizeof(Grid) == ctypes.sizeof(_lib.GridStruct)
+    for (name, _), (name2, _) in zip(Grid._fields_, _lib.GridStruct._fields_):
+        assert getattr(Grid, name).offset == getattr(_lib.GridStruct, name2).offset, name

@@ -37,18 +37,35 @@ def hipcc() -> str:
     return path
 
 
-def build(verbose: bool = False) -> Path:
-    """Compile every HIP source into warp_sph_gpt_amd/libsphdem.so."""
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", str(tmp), *[str(CSRC / s) for s in SOURCES]]
+def _run(cmd, verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        raise RuntimeError(f"hipcc failed with exit code {res.returncode}: {cmd[-1]}")
     if verbose and res.stderr.strip():
         sys.stderr.write(res.stderr)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every HIP source into warp_sph_gpt_amd/libsphdem.so: one
+    object per source, compiled in parallel, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    objs = [objdir / (Path(s).stem + ".o") for s in SOURCES]
+    cmds = [[hipcc(), *compile_flags, "-c", "-o", str(o), str(CSRC / s)]
+            for s, o in zip(SOURCES, objs)]
+    workers = jobs or min(len(cmds), max(1, (os.cpu_count() or 2) // 2), 8)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
+          *[str(o) for o in objs]], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -166,6 +166,9 @@ __global__ void k_step_dem(DemV<T> D, T dt) {
 
 // Copy a finished CSR (offsets n+1, E = off[n] edges) into the read side:
 // the no-round-trip step keeps buffer addresses fixed instead of swapping.
+// Offsets are clamped to the capacity, so after an overflow (flags bit 0,
+// which the caller must act on) the next step's history lookups stay inside
+// the cap entries the read side holds.
 template <typename T>
 __global__ void k_copy_csr(const int *off_s, const int *id_s, const T *tang_s, int *off_d,
                            int *id_d, T *tang_d, int n, int cap) {
@@ -173,7 +176,7 @@ __global__ void k_copy_csr(const int *off_s, const int *id_s, const T *tang_s, i
     const int lim = E < cap ? E : cap;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < (n + 1 > cap ? n + 1 : cap);
          k += gridDim.x * blockDim.x) {
-        if (k <= n) off_d[k] = off_s[k];
+        if (k <= n) off_d[k] = min(off_s[k], cap);
         if (k < lim) {
             id_d[k] = id_s[k];
             tang_d[3 * k] = tang_s[3 * k];
@@ -941,9 +944,11 @@ __global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> 
         const int cap = kind ? D.cap_bnd_new : D.cap_dem_new;
         __shared__ int total;
         block_scan_rows<T>(cnt + row(fin, kind, 0), n, off_n, &total);
+        // the read side's offsets are clamped to the capacity (as k_copy_csr
+        // clamps them), so an overflow leaves the next lookups in bounds
         for (int a = threadIdx.x; a < n; a += kSubThreads) {
             const int o = off_n[a];
-            off_o[a] = o;
+            off_o[a] = min(o, cap);
             const int r = row(fin, kind, a);
             for (int k = 0; k < cnt[r]; ++k) {
                 if (o + k >= cap) {
@@ -959,7 +964,7 @@ __global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> 
             }
         }
         if (threadIdx.x == 0) {
-            off_o[n] = total;
+            off_o[n] = min(total, cap);
             off_n[n] = total;
         }
         __syncthreads();
@@ -1133,6 +1138,17 @@ int sphdem_interaction_sph(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_dem
     DISPATCH(p->precision,
              (launch_fluid_final<T, true, false>(p, P, D, gs, gd, support, as_stream(stream))));
     return check_launch("interaction_sph");
+}
+
+int sphdem_device_init(void *stream) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(as_stream(stream), &cap) != hipSuccess)
+        return fail(SPHDEM_E_HIP, "stream capture status query failed");
+    SPHDEM_REQUIRE(cap == hipStreamCaptureStatusNone,
+                   "sphdem_device_init must not run under graph capture");
+    if (!side_stream(as_stream(stream)))
+        return fail(SPHDEM_E_HIP, "could not create the side stream and its events");
+    return SPHDEM_OK;
 }
 
 int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,

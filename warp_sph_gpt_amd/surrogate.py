@@ -541,6 +541,9 @@ class SurrogateRollout:
         self._dem_scratch = torch.zeros(
             max(int(_lib.load().sphdem_dem_substeps_scratch(nd, 4)), 1), dtype=torch.uint8,
             device=e.device)
+        # the DEM substeps' own flags (bit 0 of the engine's belongs to its
+        # overflow guard)
+        self._flags = torch.zeros(4, dtype=torch.int32, device=e.device)
 
     # ---- one call of each kind
     def _enqueue_surrogate(self, step: int) -> None:
@@ -555,9 +558,10 @@ class SurrogateRollout:
                   dem, e.gs.struct(), e.gb.struct(), e.gd.struct() if e.coupled else None,
                   int(step), e.ws.struct(), st)
         if e.coupled:
+            ws = e.ws.struct()
+            ws.flags = self._flags.data_ptr()
             _lib.call("sphdem_dem_substeps", e.params, dem, _lib.dembnd_view(e.W), e.gd.struct(),
-                      e.gw.struct(), SUBSTEPS, self.skin, _lib.fptr(self._dem_scratch),
-                      e.ws.struct(), st)
+                      e.gw.struct(), SUBSTEPS, self.skin, _lib.fptr(self._dem_scratch), ws, st)
         out = self.net.forward(e.params, e.P, e.gs, e.gb)
         _lib.call("sphdem_mpnn_apply", e.params, _lib.sph_view(e.P), e.gs.struct(),
                   _lib.fptr(out), self.T, st)
@@ -636,7 +640,7 @@ class SurrogateRollout:
     def _dem_flags(self) -> bool:
         """True (and cleared) when a DEM substep overflowed a contact buffer
         or a DEM outran the skin since the last look: those calls are invalid."""
-        f = self.eng.ws.flags
+        f = self._flags
         v = int(f[0].item())
         if v & 7:
             f[0] = 0
