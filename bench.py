@@ -33,6 +33,13 @@ import time
 import torch
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# FP32 operations of one porosity term (KERNEL_SPHDEM_interaction.py:80-83
+# with Kernel_KNL.py:17-24), the per-subject constant 21/(16 pi h^3) taken
+# as one multiply: the difference and its length (3 sub, 3 mul, 2 add,
+# sqrt), q = r/h (1), u = 1 - q/2 (2), u^4 (2), 1 + 2q (2), C u^4 (1 + 2q)
+# (2), m/rho (1), V W (1) and the sum (1)
+POROSITY_FLOP_PER_PAIR = 21
 
 # Algorithmic bytes per launch of each pass: (per fluid, per wall, per DEM).
 PASS_BYTES = {
@@ -73,6 +80,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-stride", type=int, default=None,
                     help="oracle subject stride (default 8 for c2, else 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph-leg", action="store_true",
+                    help="skip the hipGraph replay timing after the eager timed region")
     ap.add_argument("--traffic", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                        "profiles", "porosity_c2_traffic.json"),
                     help="JSON file with measured HBM bytes per launch of the dominant pass "
@@ -139,6 +148,22 @@ def main(argv=None) -> int:
     diag = eng.diagnostics()
     log(f"[bench] {steps} steps in {elapsed:.3f}s; per-pass ms: "
         + ", ".join(f"{k}={v:.1f}" for k, v in timer.ms.items() if v > 0))
+    # the same number of steps again as hipGraph replays (no per-pass events)
+    graph_run = None
+    if not args.graph and not args.no_graph_leg:
+        eng.capture()
+        eng.run(2)
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        eng.run(steps)
+        torch.cuda.synchronize()
+        g_el = time.perf_counter() - start
+        eng.check()
+        graph_run = {"ms_per_step": round(g_el / steps * 1e3, 4),
+                     "value": round(P.n * steps / g_el, 1), "steps": steps,
+                     "note": "same engine, one hipGraph replay per step after the eager "
+                             "timed region; value above is the eager run's"}
+        log(f"[bench] {steps} graph steps in {g_el:.3f}s")
 
     value = P.n * steps / elapsed
     dom = max(timer.ms, key=lambda k: timer.ms[k])
@@ -147,35 +172,54 @@ def main(argv=None) -> int:
     bf, bb, bd = PASS_BYTES[dom]
     bytes_launch = bf * P.n + bb * B.n + bd * D.n
     achieved = bytes_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    traffic = None
+    here = os.path.dirname(os.path.abspath(__file__))
+    traffic = traffic_src = None
     if args.traffic and os.path.exists(args.traffic):
         with open(args.traffic) as f:
             t = json.load(f)
         if t.get("kernel") == dom and t.get("workload") == wl.name:
             traffic = int(t["bytes_per_launch"])
+            traffic_src = os.path.relpath(args.traffic, here)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "traffic_source": traffic_src,
                 "kernel": dom, "avg_launch_ms": round(avg_s * 1e3, 4),
                 "bytes_per_launch": int(bytes_launch)}
-    # pair rate of the dominant pass: pairs per launch from its counter
-    # profile (16 x 16 pairs per matrix product) over this run's launch time
+    # pair rate and FP32 rate of the dominant pass: pairs per launch from its
+    # counter profile (16 x 16 pairs per matrix product) over this run's
+    # launch time, at the reference term's flops per pair
     counters = None
     if args.counters and os.path.exists(args.counters) and avg_s > 0:
         with open(args.counters) as f:
             c = json.load(f)
         if c.get("kernel") == dom and c.get("workload") == wl.name and c.get("math") == args.math:
-            counters = {"pairs_evaluated_per_launch": c["pairs_evaluated_per_launch"],
-                        "pair_rate": round(c["pairs_evaluated_per_launch"] / avg_s, 1),
-                        "valu_issue_frac": c["valu_issue_frac"],
-                        "source": os.path.relpath(args.counters,
-                                                  os.path.dirname(os.path.abspath(__file__)))}
-            # the pass is bound by vector-ALU issue, not by HBM bytes
-            roofline["limiter"] = {"what": "valu_issue", "frac": c["valu_issue_frac"]}
+            src = os.path.relpath(args.counters, here)
+            pairs = c["pairs_evaluated_per_launch"]
+            counters = {"pairs_evaluated_per_launch": pairs,
+                        "pair_rate": round(pairs / avg_s, 1),
+                        "valu_issue_frac": c["valu_issue_frac"], "source": src}
+            # the pass is bound by vector-ALU issue, not by HBM bytes: its
+            # FP32 rate against the vector FP32 peak, and the issue fraction
+            tflops = pairs * POROSITY_FLOP_PER_PAIR / avg_s / 1e12
+            roofline["limiter"] = {"what": "valu_issue", "frac": c["valu_issue_frac"],
+                                   "source": src}
+            roofline["valu"] = {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                                "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                                "flop_per_pair": POROSITY_FLOP_PER_PAIR,
+                                "pairs_per_launch": pairs, "issue_frac": c["valu_issue_frac"],
+                                "source": src}
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
         stride = args.cpu_stride or (8 if wl.name == "c2" else 1)
         cpu = cpu_baseline(wl, args.cpu_steps, stride)
+        from oracle import oracle as O
+        threads = O.num_threads()
+        O.set_num_threads(1)
+        try:
+            cpu1 = cpu_baseline(wl, args.cpu_steps, 64 if wl.name == "c2" else stride)
+        finally:
+            O.set_num_threads(threads)
     sur = None
     if args.surrogate_sim_time > 0:
         del eng
@@ -192,7 +236,8 @@ def main(argv=None) -> int:
                    "wall": B.n, "dem": D.n, "dem_wall": W.n, "dt": wl.solv.dt,
                    "graph": bool(args.graph), "math": args.math, "tuning": tuned,
                    "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "surrogate": sur,
+        "roofline": roofline, "cpu_baseline": cpu, "cpu_baseline_1thread": cpu1,
+        "graph": graph_run, "surrogate": sur,
         "dominant_pass_counters": counters,
         "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
         # these run on a second stream beside the SPH passes and the porosity:

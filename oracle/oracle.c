@@ -45,3 +45,13 @@ int ora_num_threads(void) {
     return 1;
 #endif
 }
+
+/* Threads of the OpenMP loops from here on (the baseline's 1-thread leg). */
+void ora_set_num_threads(int n) {
+#ifdef _OPENMP
+    extern void omp_set_num_threads(int);
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}

@@ -103,6 +103,11 @@ def num_threads() -> int:
     return int(lib().c.ora_num_threads())
 
 
+def set_num_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP loops from here on."""
+    lib().c.ora_set_num_threads(int(n))
+
+
 class OStruct:
     """numpy arrays under the reference's field names."""
 

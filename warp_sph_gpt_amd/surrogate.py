@@ -600,11 +600,14 @@ class SurrogateRollout:
         return [(s, name, getattr(s, name).clone()) for s in (e.P, e.B, e.D)
                 if s is not None for name, _ in s.FIELDS if getattr(s, name) is not None]
 
-    def _density(self, pos: torch.Tensor, out: torch.Tensor) -> None:
-        """Shepard-normalised density of the fluid at `pos` (FP32) into out."""
+    def _density(self, pos: torch.Tensor, out: torch.Tensor, P=None) -> None:
+        """Shepard-normalised density of the fluid at `pos` (FP32) into out,
+        with the masses of P (default: this rollout's fluid) and the prior
+        densities held in out."""
         from .struct import SPHptl
         e, s = self.eng, self.solv
-        Pj = SPHptl(**{n: getattr(e.P, n) for n in SPHptl.field_names()})
+        src = e.P if P is None else P
+        Pj = SPHptl(**{n: getattr(src, n) for n in SPHptl.field_names()})
         Pj.pos, Pj.flt, Pj.rho = pos, torch.empty_like(e.P.flt), out
         self._gj.build(pos, s.support, weights=e.P.m)
         st = _lib.stream_handle()
@@ -653,20 +656,27 @@ class SurrogateRollout:
         the sound speed, and the Shepard density at the current positions
         deviates from rho0 by less than health_tol in RMS and 15 % at most
         (the plain path keeps about 0.4 % and 8 %)."""
-        e = self.eng
-        rho = e.P.rho.clone()
-        self._density(e.P.pos, rho)
-        d = rho.double() / float(self.solv.rho0) - 1.0
-        dev = float(torch.sqrt((d ** 2).mean()))
-        dmax = float(d.abs().max())
-        pos = e.P.pos
-        inside = bool(((pos >= self._box_lo) & (pos <= self._box_hi)).all())
-        vmax = float(e.P.vel.abs().max())
-        ok = (math.isfinite(dev) and dev < self.health_tol and dmax < 0.15 and inside
-              and math.isfinite(vmax) and vmax < float(self.solv.c0) / 3.0)
+        ok = self.health()["ok"]
         if not ok:
             self.stats.unhealthy += 1
         return ok
+
+    def health(self, P=None) -> dict:
+        """The quantities healthy() tests, for this rollout's fluid or for
+        another state of the same scene (e.g. the plain run's)."""
+        P = self.eng.P if P is None else P
+        rho = P.rho.clone()
+        self._density(P.pos, rho, P)
+        d = rho.double() / float(self.solv.rho0) - 1.0
+        dev = float(torch.sqrt((d ** 2).mean()))
+        dmax = float(d.abs().max())
+        pos = P.pos
+        inside = bool(((pos >= self._box_lo) & (pos <= self._box_hi)).all())
+        vmax = float(P.vel.abs().max())
+        ok = (math.isfinite(dev) and dev < self.health_tol and dmax < 0.15 and inside
+              and math.isfinite(vmax) and vmax < float(self.solv.c0) / 3.0)
+        return {"density_rms": dev, "density_max": dmax, "inside": inside, "vmax": vmax,
+                "ok": ok}
 
     # ---- loop
     def _checkpoint(self) -> None:
@@ -811,7 +821,8 @@ def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
            "density_err_mean": float(np.mean(st.density_err)) if st.density_err else None,
            "velocity_err_max": max(st.velocity_err) if st.velocity_err else None,
            "velocity_err_mean": float(np.mean(st.velocity_err)) if st.velocity_err else None,
-           "final_plain": d_p, "final_surrogate": d_s}
+           "final_plain": d_p, "final_surrogate": d_s, "health_end": ro.health(),
+           "health_plain_end": ro.health(plain.P)}
     if log:
         log(f"[surrogate] rollout: {out}")
     return out
