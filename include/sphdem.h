@@ -163,6 +163,17 @@ int sphdem_scan_scratch(int n);
  * inclusive=False), source/Simulation.py:148-149, 181-182. */
 int sphdem_exclusive_scan(int *a, int n, int *scratch, void *stream);
 
+/* No reference counterpart: sphdem_exclusive_scan for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The tile totals live in a device array
+ * the library holds per GPU until the process ends (32,769 ints, resolved on
+ * the device's first call or by sphdem_device_init) instead of a caller's
+ * scratch, so n is at most SPHDEM_SCAN_NOALLOC_MAX (larger: SPHDEM_E_ARG,
+ * nothing enqueued).  It allocates nothing.  Every call on one GPU shares the
+ * array: calls that overlap on a GPU (other streams or threads) race. */
+#define SPHDEM_SCAN_NOALLOC_MAX (1 << 26)
+int sphdem_exclusive_scan_noalloc(int *a, int n, void *stream);
+
 /* wp.HashGrid.build(points, radius) (main.py:66,73,85,87): bucket keys with
  * 1/radius cells folded into dim_x*dim_y*dim_z buckets, a stable counting
  * sort of the points by key, bucket ranges, and a sorted copy of the points
@@ -248,9 +259,10 @@ int sphdem_sph_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
                     const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step,
                     int integrate, const sphdem_work_t *w, void *stream);
 
-/* Per-device set-up of the fused coupled step (no reference counterpart;
- * Warp creates its streams on wp.init()): the side stream and events the
- * DEM contact work runs on, for the current device.  Call once per device
+/* Per-device set-up (no reference counterpart; Warp creates its streams on
+ * wp.init()): the side stream and events the fused step's DEM contact work
+ * runs on, and the device address of sphdem_exclusive_scan_noalloc's array,
+ * for the current device.  Call once per device
  * before capturing a coupled step into a hipGraph, outside capture; a first
  * step outside capture does the same.  Until then a captured step records a
  * single-stream step (the same results, without the overlap). */

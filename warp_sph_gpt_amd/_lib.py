@@ -172,6 +172,7 @@ _SIGNATURES = {
     "sphdem_tuning_set": ([POINTER(TuningStruct)], c_int),
     "sphdem_exclusive_scan": ([c_void_p, c_int, c_void_p, c_void_p], c_int),
     "sphdem_device_init": ([c_void_p], c_int),
+    "sphdem_exclusive_scan_noalloc": ([c_void_p, c_int, c_void_p], c_int),
     "sphdem_grid_build": ([POINTER(GridStruct), c_void_p, c_int, c_double, c_void_p, c_void_p], c_int),
     "sphdem_grid_set_w": ([POINTER(GridStruct), c_void_p, c_void_p], c_int),
     "sphdem_grid_warp_bounds": ([POINTER(GridStruct), c_void_p, c_void_p, c_void_p], c_int),

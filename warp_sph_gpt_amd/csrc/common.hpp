@@ -28,6 +28,11 @@ int check_launch(const char *what);
 // Device-wide exclusive scan (grid.hip); `scratch` holds scan_scratch(n) ints.
 int scan_scratch(int n);
 int exclusive_scan(const int *in, int *out, int n, int *scratch, hipStream_t st);
+// Devices the library keeps per-device state for.
+constexpr int kMaxDevices = 64;
+// Device address of sphdem_exclusive_scan_noalloc's tile totals on the
+// current device, resolved on the first call per device (grid.hip).
+int noalloc_scan_scratch(int **out);
 
 // Per-pass event timing (grid.hip); a no-op unless sphdem_profile_enable(1).
 int prof_begin(int pass, hipStream_t st);

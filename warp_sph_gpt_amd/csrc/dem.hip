@@ -1148,7 +1148,7 @@ int sphdem_device_init(void *stream) {
                    "sphdem_device_init must not run under graph capture");
     if (!side_stream(as_stream(stream)))
         return fail(SPHDEM_E_HIP, "could not create the side stream and its events");
-    return SPHDEM_OK;
+    return noalloc_scan_scratch(nullptr);
 }
 
 int sphdem_coupled_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,

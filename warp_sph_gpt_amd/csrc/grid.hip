@@ -14,6 +14,7 @@
 #include "common.hpp"
 
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 namespace sphdem {
@@ -148,6 +149,36 @@ __global__ void k_scan_add(int *out, int n, const int *partial) {
 }
 
 int scan_scratch(int n) { return blocks_for(n, kScanTile) + 1; }
+
+// ---------------------------------------------------------------- scan without allocation
+// The tile totals of sphdem_exclusive_scan_noalloc: one array per GPU, which
+// the GPU holds until the process ends and every call on it shares.  In host
+// code `noalloc_scratch` only names the symbol; its device address on each
+// GPU comes from hipGetSymbolAddress, resolved once per device.
+constexpr int kNoallocScanInts = (SPHDEM_SCAN_NOALLOC_MAX + kScanTile - 1) / kScanTile + 1;
+static __device__ __align__(256) int noalloc_scratch[kNoallocScanInts];
+
+namespace {
+std::mutex g_noalloc_mu;
+int *g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scan_scratch(int **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(SPHDEM_E_HIP, "no current HIP device for the scan scratch");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !p)
+            return fail(SPHDEM_E_HIP, std::string("hipGetSymbolAddress(noalloc_scratch): ") +
+                                          hipGetErrorString(e));
+        g_noalloc_ptr[dev] = static_cast<int *>(p);
+    }
+    if (out) *out = g_noalloc_ptr[dev];
+    return SPHDEM_OK;
+}
 
 int exclusive_scan(const int *in, int *out, int n, int *scratch, hipStream_t st) {
     if (n <= 0) return SPHDEM_OK;
@@ -384,6 +415,19 @@ int sphdem_exclusive_scan(int *a, int n, int *scratch, void *stream) {
     SPHDEM_REQUIRE(n >= 0, "scan length must be nonnegative");
     SPHDEM_REQUIRE(n == 0 || (a && scratch), "scan buffers are NULL");
     return exclusive_scan(a, a, n, scratch, as_stream(stream));
+}
+
+int sphdem_exclusive_scan_noalloc(int *a, int n, void *stream) {
+    SPHDEM_REQUIRE(n >= 0, "scan length must be nonnegative");
+    SPHDEM_REQUIRE(n <= SPHDEM_SCAN_NOALLOC_MAX,
+                   "scan length exceeds SPHDEM_SCAN_NOALLOC_MAX (" +
+                       std::to_string(SPHDEM_SCAN_NOALLOC_MAX) + "): use sphdem_exclusive_scan");
+    SPHDEM_REQUIRE(n == 0 || a, "scan buffer is NULL");
+    if (n == 0) return SPHDEM_OK;
+    int *tiles = nullptr;
+    const int rc = noalloc_scan_scratch(&tiles);
+    if (rc) return rc;
+    return exclusive_scan(a, a, n, tiles, as_stream(stream));
 }
 
 int sphdem_grid_build(sphdem_grid_t *g, const void *points, int n, double radius, const void *w,
