@@ -226,3 +226,64 @@ def test_parsed_fields_are_packed_row_major():
         got = input_parser(GOLDEN / "reference_input_head" / f"input_{kind}.txt")
         for name, _ in got.FIELDS:
             assert getattr(got, name).is_contiguous(), f"{kind}.{name}"
+
+
+@pytest.mark.parametrize("text,message", [
+    ("", r":1: empty header or duplicate column IDs"),
+    ("1 1 2 3 4 5 6 7\n", r":1: empty header or duplicate column IDs"),
+    ("1 2 3 4 5 6 7 99\n", r":1: missing IDs \[8\]; unknown IDs \[99\]"),
+    ("1.0 2 3 4 5 6 7 8\n", r":1: header must contain integer column IDs"),
+    ("1 2 3 4 5 6 7 8\n0 0 0 0 0 0 1000 0.001\n0 0\n", r":3: expected 8 values, got 2"),
+    ("1 2 3 4 5 6 7 8\n0 0 0 0 0 0 1000 0.001\n0 0 0 0 0 0 oops 0.001\n",
+     r":3: invalid numeric value"),
+    ("1 2 3 4 5 6 7 8\n0 0 0 0 0 0 1000 0.001\n0 0 0 0 0 0 nan 0.001\n",
+     r":3: values must be finite float32 numbers"),
+    ("1 2 3 4 5 6 7 8\n0 0 0 0 0 0 1e100 0.001\n", r":2: values must be finite float32 numbers"),
+    ("1 2 3 4 5 6 7 8\n0 0 0 0 0 0 -1 0.001\n", r": rho must be positive in float32"),
+    ("1 2 3 4 5 6 7 8\n0 0 0 0 0 0 1000 1e-100\n", r": m must be positive in float32"),
+])
+def test_rejections_name_the_line_and_the_rule(tmp_path, text, message):
+    """The messages of input/input_reader.py:48-95: the offending line of a
+    row error, the field of a value error."""
+    path = tmp_path / "input_SPH.txt"
+    path.write_text(text)
+    with pytest.raises(ValueError, match=message):
+        input_parser(path)
+
+
+def test_sphere_rules(tmp_path):
+    head = " ".join(str(i) for i in range(1, 18))
+    ok = "0 0 0 0 0 0 0 0 0 0.025 2500 0.16362461737446837 1e-5 2e4 4 0.3 0.08"
+    bad_mass = ok.replace("0.16362461737446837", "0.2")
+    bad_eta = ok.replace(" 4 0.3", " -4 0.3")
+    path = tmp_path / "input_DEM.txt"
+    path.write_text(head + "\n" + ok + "\n")
+    assert input_parser(path).n == 1
+    for row, message in ((bad_mass, "sphere mass must equal"), (bad_eta, "eta must be nonnegative")):
+        path.write_text(head + "\n" + row + "\n")
+        with pytest.raises(ValueError, match=message):
+            input_parser(path)
+
+
+def test_generator_rules_name_the_first_broken_one():
+    """input_gen/config.py:227-353: rules in order, each with its message."""
+    from warp_sph_gpt_amd.input_gen import GenerationConfig
+    cases = [
+        (CoupledGenerationConfig(dx=-0.02).validate_sph, "dx must be finite and positive"),
+        (CoupledGenerationConfig(dt=1.0e-3).validate_sph, "acoustic limit"),
+        (CoupledGenerationConfig(tank_width=2.01).validate_sph, "tank_width must be an integer"),
+        (CoupledGenerationConfig(fluid_width=2.5).validate_sph,
+         "initial fluid block must fit the tank along x"),
+        (CoupledGenerationConfig(dem_eta=-1.0).validate_dem, "dem_eta and dem_mu"),
+        (CoupledGenerationConfig(dem_spacing=0.04).validate_dem, "must not overlap"),
+        (CoupledGenerationConfig(dem_porosity_min=0.0).validate_dem, "porosity bounds"),
+        (CoupledGenerationConfig(dt=5.0e-4, c0=1.0, h=10.0).validate_dem, "DEM contact limit"),
+        (GenerationConfig(dx=1.0 / 1024, dt=2.0e-5).validate_eisph, "viscosity limit"),
+        (GenerationConfig(dx=1.0 / 25, dt=0.02).validate_eisph, "advection limit"),
+    ]
+    for check, message in cases:
+        with pytest.raises(ValueError, match=message):
+            check()
+    GenerationConfig(dx=1.0 / 1024, dt=1.0e-5).validate_eisph()
+    CoupledGenerationConfig().validate_sph()
+    CoupledGenerationConfig().validate_dem()

@@ -48,50 +48,61 @@ class BaseSolv:
         """Wendland C2 support radius, 2*h (input/Config.py:47-50)."""
         return 2.0 * self.h
 
+    # ---- validation: each rule in the reference's order (the first broken
+    # one names itself); `exact` integers exclude bool, as type(v) is int does
     def _validate_common(self) -> None:
-        for name in ("h", "rho0", "dt"):
-            value = getattr(self, name)
-            if not math.isfinite(value) or value <= 0.0:
-                raise ValueError(f"{name} must be finite and positive")
-        for name in ("n_steps", "output_step", "shepard_step"):
-            value = getattr(self, name)
-            if type(value) is not int or value < 0:
-                raise ValueError(f"{name} must be a nonnegative integer")
-        for name in ("grid_slice", "gif_fps"):
-            value = getattr(self, name)
-            if type(value) is not int or value < 1:
-                raise ValueError(f"{name} must be a positive integer")
+        positive(self, "h", "rho0", "dt")
+        integers(self, 0, "a nonnegative integer", "n_steps", "output_step", "shepard_step",
+                 exact=True)
+        integers(self, 1, "a positive integer", "grid_slice", "gif_fps", exact=True)
 
     def validate_sph(self) -> None:
         """input/Config.py:65-78: solver settings and the acoustic dt limit."""
         self._validate_common()
-        for name in ("c0", "gamma"):
-            value = getattr(self, name)
-            if not math.isfinite(value) or value <= 0.0:
-                raise ValueError(f"{name} must be finite and positive")
-        for name in ("mu", "g"):
-            value = getattr(self, name)
-            if not math.isfinite(value) or value < 0.0:
-                raise ValueError(f"{name} must be finite and nonnegative")
-        if self.dt > 0.25 * self.h / self.c0:
-            raise ValueError("dt exceeds the initial SPH acoustic limit 0.25*h/c0")
+        positive(self, "c0", "gamma")
+        nonnegative(self, "mu", "g")
+        rule(self.dt <= 0.25 * self.h / self.c0,
+             "dt exceeds the initial SPH acoustic limit 0.25*h/c0")
         self.validate_dem()
 
     def validate_eisph(self) -> None:
         """input/Config.py:89-93: EISPH controls; the loaded points supply the
         geometry and the wall speeds."""
         self._validate_common()
-        if not math.isfinite(self.nu) or self.nu < 0.0:
-            raise ValueError("nu must be finite and nonnegative")
+        nonnegative(self, "nu")
 
     def validate_dem(self) -> None:
         """input/Config.py:80-87: coupling controls."""
         if not self.dem_enable:
             return
-        if not 0.0 < self.dem_porosity_min <= self.dem_porosity_max <= 1.0:
-            raise ValueError("DEM porosity bounds must satisfy 0 < min <= max <= 1")
-        if not math.isfinite(self.dem_dt_safety) or self.dem_dt_safety <= 0.0:
-            raise ValueError("dem_dt_safety must be finite and positive")
+        rule(0.0 < self.dem_porosity_min <= self.dem_porosity_max <= 1.0,
+             "DEM porosity bounds must satisfy 0 < min <= max <= 1")
+        positive(self, "dem_dt_safety")
+
+
+def rule(holds: bool, message: str) -> None:
+    """Raise ValueError(message) unless the rule holds."""
+    if not holds:
+        raise ValueError(message)
+
+
+def positive(cfg, *names) -> None:
+    for name in names:
+        v = getattr(cfg, name)
+        rule(math.isfinite(v) and v > 0.0, f"{name} must be finite and positive")
+
+
+def nonnegative(cfg, *names) -> None:
+    for name in names:
+        v = getattr(cfg, name)
+        rule(math.isfinite(v) and v >= 0.0, f"{name} must be finite and nonnegative")
+
+
+def integers(cfg, minimum: int, what: str, *names, exact: bool = False) -> None:
+    for name in names:
+        v = getattr(cfg, name)
+        is_int = type(v) is int if exact else isinstance(v, int)
+        rule(is_int and v >= minimum, f"{name} must be {what}")
 
 
 @dataclass

@@ -17,6 +17,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
+from .config import integers, positive, rule
 from .input_reader import INPUT_FIELDS, column_indices
 from .struct import BNDptl, DEMBNDptl, DEMptl, EISPHptl, SPHptl
 
@@ -130,96 +131,88 @@ class GenerationConfig:
         """Cell-centred fluid points along one cavity side."""
         return int(round(self.length / self.dx))
 
+    # ---- validation (input_gen/config.py:227-353): the same rules, checked
+    # in the same order, so that the first broken rule names itself
     def validate_eisph(self) -> None:
-        """input_gen/config.py:269-300: cavity lattice, ghost layers covering
-        the 2h support, and the explicit advection dt limit."""
-        for name in ("length", "dx", "h_factor", "h", "support", "rho0",
-                     "lid_velocity", "reynolds", "dt"):
-            value = getattr(self, name)
-            if not math.isfinite(value) or value <= 0.0:
-                raise ValueError(f"{name} must be finite and positive")
-        for name in ("bnd_layer", "n_steps", "output_step", "gif_fps", "grid_slice"):
-            value = getattr(self, name)
-            if not isinstance(value, int) or value < 1:
-                raise ValueError(f"{name} must be a positive integer")
-        if not math.isclose(self.h, self.h_factor * self.dx):
-            raise ValueError("EISPH smoothing length must equal h_factor*dx")
-        if not math.isclose(self.support, 2.0 * self.h):
-            raise ValueError("Wendland support must equal 2*h")
-        if self.cells < 5 or not math.isclose(self.cells * self.dx, self.length,
-                                               rel_tol=0.0, abs_tol=1.0e-10):
-            raise ValueError("length must be an integer multiple of dx with at least 5 cells")
-        if self.bnd_layer < math.ceil(self.support / self.dx):
-            raise ValueError("bnd_layer must cover the complete kernel support")
-        if self.bnd_layer > self.cells:
-            raise ValueError("bnd_layer cannot exceed the cavity cell count")
-        if self.dt > 0.25 * self.dx / self.lid_velocity:
-            raise ValueError("dt exceeds the explicit advection limit")
+        """The cavity lattice, ghost layers covering the 2h support, and the
+        explicit predictor's advection and viscosity dt limits
+        (input_gen/config.py:269-305)."""
+        positive(self, "length", "dx", "h_factor", "h", "support", "rho0", "lid_velocity",
+                  "reynolds", "dt")
+        integers(self, 1, "a positive integer", "bnd_layer", "n_steps", "output_step",
+                  "gif_fps", "grid_slice")
+        rule(math.isclose(self.h, self.h_factor * self.dx),
+              "EISPH smoothing length must equal h_factor*dx")
+        rule(math.isclose(self.support, 2.0 * self.h), "Wendland support must equal 2*h")
+        rule(self.cells >= 5 and math.isclose(self.cells * self.dx, self.length, rel_tol=0.0,
+                                               abs_tol=1.0e-10),
+              "length must be an integer multiple of dx with at least 5 cells")
+        rule(self.bnd_layer >= math.ceil(self.support / self.dx),
+              "bnd_layer must cover the complete kernel support")
+        rule(self.bnd_layer <= self.cells, "bnd_layer cannot exceed the cavity cell count")
+        rule(self.dt <= 0.25 * self.dx / self.lid_velocity,
+              "dt exceeds the explicit advection limit")
+        rule(self.dt <= 0.125 * self.dx * self.dx / self.nu,
+              "dt exceeds the explicit viscosity limit")
 
     def validate_sph(self) -> None:
-        """input_gen/config.py:227-263: domain, lattice and dt checks."""
-        for name in ("dx", "h", "support", "tank_width", "tank_height", "tank_depth",
-                     "fluid_width", "fluid_height", "fluid_depth", "rho0", "c0", "gamma", "dt"):
-            value = getattr(self, name)
-            if not math.isfinite(value) or value <= 0.0:
-                raise ValueError(f"{name} must be finite and positive")
-        if not math.isclose(self.support, 2.0 * self.h):
-            raise ValueError("Wendland support must equal 2*h")
-        if self.dt > 0.25 * self.h / self.c0:
-            raise ValueError("dt exceeds the initial SPH acoustic limit 0.25*h/c0")
+        """The dam-break domain, wall lattice, material and the initial
+        acoustic dt limit (input_gen/config.py:227-263)."""
+        positive(self, "dx", "h", "support", "tank_width", "tank_height", "tank_depth",
+                  "fluid_width", "fluid_height", "fluid_depth", "rho0", "c0", "gamma", "dt")
+        rule(math.isclose(self.support, 2.0 * self.h), "Wendland support must equal 2*h")
+        rule(self.dt <= 0.25 * self.h / self.c0,
+              "dt exceeds the initial SPH acoustic limit 0.25*h/c0")
         for name in ("g", "mu"):
-            if not math.isfinite(getattr(self, name)) or getattr(self, name) < 0.0:
-                raise ValueError(f"{name} must be finite and nonnegative")
-        for name in ("bnd_layer", "grid_slice"):
-            if not isinstance(getattr(self, name), int) or getattr(self, name) < 1:
-                raise ValueError(f"{name} must be a positive integer")
-        for name in ("n_steps", "output_step", "shepard_step"):
-            if not isinstance(getattr(self, name), int) or getattr(self, name) < 0:
-                raise ValueError(f"{name} must be a nonnegative integer")
+            v = getattr(self, name)
+            rule(math.isfinite(v) and v >= 0.0, f"{name} must be finite and nonnegative")
+        integers(self, 1, "a positive integer", "bnd_layer", "grid_slice")
+        integers(self, 0, "a nonnegative integer", "n_steps", "output_step", "shepard_step")
+        # the wall lattice is integer: tank extents are whole multiples of dx
         for name in ("tank_width", "tank_height", "tank_depth"):
             cells = getattr(self, name) / self.dx
-            if not math.isclose(cells, round(cells), abs_tol=1.0e-8):
-                raise ValueError(f"{name} must be an integer multiple of dx")
-        for axis, extent, tank in (("x", self.fluid_width, self.tank_width),
-                                   ("y", self.fluid_depth, self.tank_depth),
-                                   ("z", self.fluid_height, self.tank_height)):
-            origin = getattr(self, "fluid_origin_" + axis)
-            if (not math.isfinite(origin) or origin < 0.0 or origin + extent > tank + 1.0e-12
-                    or round(extent / self.dx) < 1):
-                raise ValueError(f"initial fluid block must fit the tank along {axis}")
+            rule(math.isclose(cells, round(cells), abs_tol=1.0e-8),
+                  f"{name} must be an integer multiple of dx")
+        blocks = {"x": (self.fluid_width, self.tank_width),
+                  "y": (self.fluid_depth, self.tank_depth),
+                  "z": (self.fluid_height, self.tank_height)}
+        for axis, (extent, tank) in blocks.items():
+            lo = getattr(self, f"fluid_origin_{axis}")
+            fits = (math.isfinite(lo) and lo >= 0.0 and lo + extent <= tank + 1.0e-12
+                    and round(extent / self.dx) >= 1)
+            rule(fits, f"initial fluid block must fit the tank along {axis}")
 
     def validate_dem(self) -> None:
-        """input_gen/config.py:307-353: DEM material, spacing and contact dt."""
+        """DEM material, initial spacing, wall pores, porosity bounds and the
+        contact dt limit of two identical moving spheres
+        (input_gen/config.py:307-353); nothing to check without DEM."""
         if not self.dem_enable:
             return
-        for name in ("dem_radius", "dem_rho", "dem_K", "dem_inertia_factor",
-                     "dem_bnd_radius", "dem_bnd_rho", "dem_bnd_spacing",
-                     "dem_h", "dem_dt_safety", "dt"):
-            value = getattr(self, name)
-            if not math.isfinite(value) or value <= 0.0:
-                raise ValueError(f"{name} must be finite and positive")
-        if (not math.isfinite(self.dem_eta) or not math.isfinite(self.dem_mu)
-                or self.dem_eta < 0.0 or self.dem_mu < 0.0):
-            raise ValueError("dem_eta and dem_mu must be nonnegative")
-        if any(not isinstance(n, int) or n < 1 for n in (self.dem_nx, self.dem_ny, self.dem_nz)):
-            raise ValueError("dem_nx, dem_ny and dem_nz must be positive integers")
-        if not math.isfinite(self.dem_spacing) or self.dem_spacing < 2.0 * self.dem_radius:
-            raise ValueError("initial DEM particles must not overlap")
-        if self.dem_bnd_spacing > 2.0 * self.dem_bnd_radius:
-            raise ValueError("DEM boundary spacing must not exceed the boundary diameter")
-        if self.dem_bnd_spacing / math.sqrt(2.0) >= self.dem_radius + self.dem_bnd_radius:
-            raise ValueError("DEM boundary pores must be smaller than the moving spheres")
-        for name in ("dem_origin_x", "dem_origin_y", "dem_origin_z", "dem_vel_x",
-                     "dem_vel_y", "dem_vel_z", "dem_omega_x", "dem_omega_y", "dem_omega_z"):
-            if not math.isfinite(getattr(self, name)):
-                raise ValueError(f"{name} must be finite")
-        if not 0.0 < self.dem_porosity_min <= self.dem_porosity_max <= 1.0:
-            raise ValueError("DEM porosity bounds must satisfy 0 < min <= max <= 1")
-        mass_eff = 0.5 * self.dem_mass
-        dt_contact = self.dem_dt_safety * min(math.sqrt(mass_eff / self.dem_K),
-                                              mass_eff / max(self.dem_eta, 1.0e-20))
-        if self.dt > dt_contact:
-            raise ValueError(f"dt exceeds DEM contact limit {dt_contact:.6g} s")
+        positive(self, "dem_radius", "dem_rho", "dem_K", "dem_inertia_factor",
+                  "dem_bnd_radius", "dem_bnd_rho", "dem_bnd_spacing", "dem_h", "dem_dt_safety",
+                  "dt")
+        eta, mu = self.dem_eta, self.dem_mu
+        rule(math.isfinite(eta) and math.isfinite(mu) and eta >= 0.0 and mu >= 0.0,
+              "dem_eta and dem_mu must be nonnegative")
+        counts = (self.dem_nx, self.dem_ny, self.dem_nz)
+        rule(all(isinstance(c, int) and c >= 1 for c in counts),
+              "dem_nx, dem_ny and dem_nz must be positive integers")
+        rule(math.isfinite(self.dem_spacing) and self.dem_spacing >= 2.0 * self.dem_radius,
+              "initial DEM particles must not overlap")
+        rule(self.dem_bnd_spacing <= 2.0 * self.dem_bnd_radius,
+              "DEM boundary spacing must not exceed the boundary diameter")
+        # a moving centre must not pass a square pore of the wall lattice
+        rule(self.dem_bnd_spacing / math.sqrt(2.0) < self.dem_radius + self.dem_bnd_radius,
+              "DEM boundary pores must be smaller than the moving spheres")
+        for name in ("dem_origin_x", "dem_origin_y", "dem_origin_z", "dem_vel_x", "dem_vel_y",
+                     "dem_vel_z", "dem_omega_x", "dem_omega_y", "dem_omega_z"):
+            rule(math.isfinite(getattr(self, name)), f"{name} must be finite")
+        rule(0.0 < self.dem_porosity_min <= self.dem_porosity_max <= 1.0,
+              "DEM porosity bounds must satisfy 0 < min <= max <= 1")
+        m_star = 0.5 * self.dem_mass
+        limit = self.dem_dt_safety * min(math.sqrt(m_star / self.dem_K),
+                                         m_star / max(self.dem_eta, 1.0e-20))
+        rule(self.dt <= limit, f"dt exceeds DEM contact limit {limit:.6g} s")
 
 
 @dataclass

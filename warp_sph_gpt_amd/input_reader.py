@@ -1,13 +1,25 @@
 """Particle input files: numeric-header column-ID text format.
 
-Mirrors input/input_reader.py:1-220: the column IDs of each kind, the
-validation and error messages of `_read_table`, `input_parser`,
-`validate_dem_particles`, `dem_search_scales`, `load_sph_particles`,
-`validate_eisph_particles` and `load_eisph_particles`.  Structs are returned
-as torch tensors on the requested device.
+Interface of input/input_reader.py:1-220, kept so that the reference's files
+and callers work unchanged: one file per particle kind, a first line of
+integer column IDs (IDs restart at 1 per kind, vec3 fields take three
+consecutive IDs, columns may come in any order), then one particle per row.
+The column IDs, the inputs that are rejected and the error messages follow
+the reference; the reader itself is organised differently:
+
+  * a `Layout` per kind maps each ID to (field, component);
+  * the body is tokenised once and converted by numpy in one pass, and only
+    when that fails is the first offending line searched for, so that the
+    error names it;
+  * value checks are a table of (fields, predicate, message);
+  * fields the file does not carry are filled by a per-kind table of
+    initialisers when the struct is built.
+
+Structs are returned as torch tensors on the requested device.
 """
 from __future__ import annotations
 
+from dataclasses import dataclass
 from pathlib import Path
 
 import numpy as np
@@ -15,6 +27,7 @@ import torch
 
 from .struct import BNDptl, DEMBNDptl, DEMptl, EISPHptl, SPHptl
 
+# (field, components) in ID order, per kind (input/input_reader.py:18-31).
 INPUT_FIELDS = {
     "SPH": (("pos", 3), ("vel", 3), ("rho", 1), ("m", 1)),
     "BND": (("pos", 3), ("vel", 3), ("rho", 1), ("m", 1)),
@@ -28,191 +41,249 @@ INPUT_FIELDS = {
 STRUCTURES = {"SPH": SPHptl, "BND": BNDptl, "DEM": DEMptl, "DEMBND": DEMBNDptl,
               "EISPH": EISPHptl, "EISPHBND": EISPHptl}
 PROJECT_ROOT = Path(__file__).resolve().parents[1]
-_F32_MAX = float(np.finfo(np.float32).max)
+_F32 = np.finfo(np.float32)
+_I32_MAX = np.iinfo(np.int32).max
+
+
+@dataclass(frozen=True)
+class Layout:
+    """Column IDs of one particle kind."""
+    kind: str
+    ids: dict            # field -> tuple of IDs
+
+    @classmethod
+    def of(cls, kind: str) -> "Layout":
+        if kind not in INPUT_FIELDS:
+            raise ValueError(f"unknown particle kind {kind!r}; choose {tuple(INPUT_FIELDS)}")
+        ids, start = {}, 1
+        for field, width in INPUT_FIELDS[kind]:
+            ids[field] = tuple(range(start, start + width))
+            start += width
+        return cls(kind, ids)
+
+    @property
+    def required(self) -> set:
+        return {i for group in self.ids.values() for i in group}
 
 
 def column_indices(kind: str) -> dict:
     """Documented numeric IDs of each input field; IDs restart at 1 per kind."""
-    if kind not in INPUT_FIELDS:
-        raise ValueError(f"unknown particle kind {kind!r}; choose {tuple(INPUT_FIELDS)}")
-    index, result = 1, {}
-    for field, width in INPUT_FIELDS[kind]:
-        result[field] = tuple(range(index, index + width))
-        index += width
-    return result
+    return dict(Layout.of(kind).ids)
 
 
-def _read_table(path: Path, kind: str) -> dict:
-    columns = column_indices(kind)
-    required = {i for ids in columns.values() for i in ids}
-    with path.open(encoding="utf-8-sig") as stream:
-        try:
-            header = [int(token) for token in stream.readline().split()]
-        except ValueError as exc:
-            raise ValueError(f"{path}:1: header must contain integer column IDs") from exc
-        if not header or len(header) != len(set(header)):
-            raise ValueError(f"{path}:1: empty header or duplicate column IDs")
-        if set(header) != required:
-            raise ValueError(f"{path}:1: missing IDs {sorted(required - set(header))}; "
-                             f"unknown IDs {sorted(set(header) - required)}")
-        rows = []
-        for line_number, line in enumerate(stream, 2):
-            tokens = line.split()
-            if len(tokens) != len(header):
-                raise ValueError(f"{path}:{line_number}: expected {len(header)} values, "
-                                 f"got {len(tokens)}")
+def _header(path: Path, line: str, layout: Layout) -> list:
+    try:
+        header = [int(tok) for tok in line.split()]
+    except ValueError as exc:
+        raise ValueError(f"{path}:1: header must contain integer column IDs") from exc
+    present = set(header)
+    if not header or len(present) < len(header):
+        raise ValueError(f"{path}:1: empty header or duplicate column IDs")
+    need = layout.required
+    if present != need:
+        raise ValueError(f"{path}:1: missing IDs {sorted(need - present)}; "
+                         f"unknown IDs {sorted(present - need)}")
+    return header
+
+
+def _body(path: Path, lines: list, width: int) -> np.ndarray:
+    """The rows as an (n, width) float64 array; errors name the first bad line."""
+    tokens = [ln.split() for ln in lines]
+    short = next((k for k, t in enumerate(tokens) if len(t) != width), None)
+    if short is not None:
+        raise ValueError(f"{path}:{short + 2}: expected {width} values, "
+                         f"got {len(tokens[short])}")
+    try:
+        data = np.array(tokens, dtype=np.float64).reshape(len(lines), width)
+    except ValueError:
+        for k, t in enumerate(tokens):
             try:
-                row = [float(token) for token in tokens]
+                [float(v) for v in t]
             except ValueError as exc:
-                raise ValueError(f"{path}:{line_number}: invalid numeric value") from exc
-            if not all(np.isfinite(row)) or any(abs(v) > _F32_MAX for v in row):
-                raise ValueError(f"{path}:{line_number}: values must be finite float32 numbers")
-            rows.append(row)
-    data = np.asarray(rows, dtype=np.float64).reshape(-1, len(header))
-    result = {}
-    for field, ids in columns.items():
-        values = data[:, [header.index(i) for i in ids]]
-        result[field] = values[:, 0] if len(ids) == 1 else values
-    for field in ("rho", "m", "radius", "inertia", "K", "h"):
-        if field in result and np.any(result[field].astype(np.float32) <= 0.0):
-            raise ValueError(f"{path}: {field} must be positive in float32")
-    for field in ("eta", "mu"):
-        if field in result and np.any(result[field] < 0.0):
-            raise ValueError(f"{path}: {field} must be nonnegative")
-    if "mirror" in result:
-        ids = result["mirror"]
-        if np.any((ids != np.floor(ids)) | (ids < 0) | (ids > np.iinfo(np.int32).max)):
-            raise ValueError(f"{path}: mirror must be a nonnegative int32 fluid row index")
-    if "radius" in result:
-        volume = (4.0 / 3.0) * np.pi * result["radius"] ** 3
-        if not np.allclose(result["m"], result["rho"] * volume, rtol=1e-5, atol=0.0):
+                raise ValueError(f"{path}:{k + 2}: invalid numeric value") from exc
+        raise
+    bad = ~(np.isfinite(data) & (np.abs(data) <= _F32.max)).all(axis=1)
+    if bad.any():
+        raise ValueError(f"{path}:{int(np.argmax(bad)) + 2}: values must be finite float32 "
+                         "numbers")
+    return data
+
+
+def _sphere_volume(radius: np.ndarray) -> np.ndarray:
+    return (4.0 / 3.0) * np.pi * radius ** 3
+
+
+# Value checks after parsing: (fields, test that must hold for every entry, message).
+_CHECKS = (
+    (("rho", "m", "radius", "inertia", "K", "h"), lambda v: v.astype(np.float32) > 0.0,
+     "{f} must be positive in float32"),
+    (("eta", "mu"), lambda v: v >= 0.0, "{f} must be nonnegative"),
+    (("mirror",), lambda v: (v == np.floor(v)) & (v >= 0) & (v <= _I32_MAX),
+     "mirror must be a nonnegative int32 fluid row index"),
+)
+
+
+def _check_values(path: Path, fields: dict) -> None:
+    for names, ok, message in _CHECKS:
+        for f in names:
+            if f in fields and not np.all(ok(fields[f])):
+                raise ValueError(f"{path}: " + message.format(f=f))
+    if "radius" in fields:
+        vol = _sphere_volume(fields["radius"])
+        if not np.allclose(fields["m"], fields["rho"] * vol, rtol=1e-5, atol=0.0):
             raise ValueError(f"{path}: sphere mass must equal rho*(4/3)*pi*radius^3")
-        v32 = volume.astype(np.float32)
-        if np.any(v32 <= 0) or not np.isfinite(v32).all():
+        vol32 = vol.astype(np.float32)
+        if not (np.isfinite(vol32).all() and np.all(vol32 > 0)):
             raise ValueError(f"{path}: derived sphere volume is outside float32 range")
-    return result
+
+
+def read_fields(path, kind: str) -> dict:
+    """Parse one input file into {field: float64 array} (vec3 fields (n, 3))."""
+    path = Path(path)
+    layout = Layout.of(kind)
+    with path.open(encoding="utf-8-sig") as stream:
+        first = stream.readline()
+        header = _header(path, first, layout)
+        data = _body(path, stream.read().splitlines(), len(header))
+    where = {cid: k for k, cid in enumerate(header)}
+    fields = {}
+    for f, ids in layout.ids.items():
+        cols = data[:, [where[i] for i in ids]]
+        fields[f] = cols[:, 0] if len(ids) == 1 else cols
+    _check_values(path, fields)
+    return fields
+
+
+# Fields a file does not carry, per kind: name -> value from (fields, n).
+_DERIVED = {
+    "SPH": {"rho_raw": lambda f, n: f["rho"], "porosity": lambda f, n: np.ones(n)},
+    "BND": {"rho_raw": lambda f, n: f["rho"]},
+    "DEM": {"porosity": lambda f, n: np.ones(n),
+            "volume": lambda f, n: _sphere_volume(f["radius"])},
+    "DEMBND": {"volume": lambda f, n: _sphere_volume(f["radius"])},
+    "EISPH": {"vel_star": lambda f, n: f["vel"], "mirror": lambda f, n: np.arange(n)},
+    "EISPHBND": {},
+}
+_INTEGER_FIELDS = ("mirror",)
+
+
+def _tensor(name: str, values: np.ndarray, n: int) -> torch.Tensor:
+    if name in _INTEGER_FIELDS:
+        return torch.from_numpy(np.asarray(values).astype(np.int32))
+    t = torch.from_numpy(np.ascontiguousarray(values, dtype=np.float32))
+    return t.reshape(n, 3) if t.ndim == 2 else t
 
 
 def input_parser(file_path, kind: str | None = None, device="cpu"):
     """Read input_<kind>.txt into a fully allocated struct (input_reader.py:98-134).
 
-    Computed fields start at zero, rho_raw copies rho, fluid and DEM porosity
-    start at one, DEM volume is (4/3)*pi*radius^3, and both contact CSR sides
-    are empty with zero offsets.  EISPH fluid rows get vel_star = vel and the
-    identity mirror; ghost mirror rows are read as int32."""
+    The kind defaults to the file name's.  Fields the file does not carry
+    start at zero except: rho_raw = rho, fluid and DEM porosity = 1, DEM
+    volume = (4/3) pi radius^3, EISPH vel_star = vel and the identity mirror.
+    Both contact CSR sides are empty with zero offsets; a header-only file
+    gives an empty struct."""
     path = Path(file_path)
     kind = path.stem.removeprefix("input_") if kind is None else kind
     if kind not in STRUCTURES:
         raise ValueError(f"unknown particle kind {kind!r}; choose {tuple(STRUCTURES)}")
-    data = _read_table(path, kind)
-    n = len(data["pos"])
+    fields = read_fields(path, kind)
+    n = int(fields["pos"].shape[0])
+    fields.update({name: make(fields, n) for name, make in _DERIVED[kind].items()})
     particle = STRUCTURES[kind].allocate(n, device="cpu")
-    for name, values in data.items():
-        if name == "mirror":
-            setattr(particle, name, torch.from_numpy(np.asarray(values).astype(np.int32)))
-            continue
-        # column selection leaves numpy in column-major order; the kernels
-        # take packed row-major (N, 3) arrays
-        t = torch.from_numpy(np.ascontiguousarray(np.asarray(values, np.float64),
-                                                  dtype=np.float32))
-        setattr(particle, name, t.reshape(n, 3) if t.ndim == 2 else t)
-    if kind in ("SPH", "BND"):
-        particle.rho_raw = particle.rho.clone()
-    if kind in ("SPH", "DEM"):
-        particle.porosity.fill_(1.0)
-    if kind in ("DEM", "DEMBND"):
-        volume = (4.0 / 3.0) * np.pi * data["radius"] ** 3
-        particle.volume = torch.from_numpy(np.asarray(volume, np.float64).astype(np.float32))
-    if kind == "EISPH":
-        particle.vel_star = particle.vel.clone()
-        particle.mirror = torch.arange(n, dtype=torch.int32)
-    return particle.to(device) if str(device) != "cpu" else particle
+    for name, values in fields.items():
+        setattr(particle, name, _tensor(name, values, n))
+    return particle if str(device) == "cpu" else particle.to(device)
 
 
 def input_directory(solv) -> Path:
-    directory = Path(solv.input_dir)
-    return directory if directory.is_absolute() else PROJECT_ROOT / directory
+    path = Path(solv.input_dir)
+    return path if path.is_absolute() else PROJECT_ROOT / path
+
+
+def _host(t) -> np.ndarray:
+    return t.detach().cpu().numpy()
 
 
 def dem_search_scales(dem, boundary) -> tuple:
     """(max DEM radius, max DEM-wall radius or 0, max h, common h or 0)
     (input_reader.py:163-168)."""
-    h = dem.h.detach().cpu().numpy()
-    rb = boundary.radius.detach().cpu().numpy()
-    return (float(dem.radius.detach().cpu().numpy().max()),
-            float(rb.max()) if boundary.pos.shape[0] else 0.0,
-            float(h.max()), float(h[0]) if np.all(h == h[0]) else 0.0)
+    h = _host(dem.h)
+    wall_r = float(_host(boundary.radius).max()) if boundary.pos.shape[0] else 0.0
+    common = float(h[0]) if (h == h[0]).all() else 0.0
+    return float(_host(dem.radius).max()), wall_r, float(h.max()), common
+
+
+def dem_contact_dt_limit(solv, dem) -> float:
+    """solv.dem_dt_safety x min(sqrt(m*/K_max), m*/eta_max) with the
+    conservative reduced mass m* = min(m)/2 (input_reader.py:142-160)."""
+    m_star = 0.5 * float(_host(dem.m).astype(np.float64).min())
+    k_max = float(_host(dem.K).astype(np.float64).max())
+    eta_max = max(float(_host(dem.eta).astype(np.float64).max()), 1e-20)
+    return solv.dem_dt_safety * min(np.sqrt(m_star / k_max), m_star / eta_max)
 
 
 def validate_dem_particles(solv, dem, boundary) -> tuple:
-    """Contact dt limit 0.1*min(sqrt(m_eff/K_max), m_eff/eta_max), m_eff = min(m)/2
-    (input_reader.py:142-160); returns dem_search_scales."""
+    """Reject a time step above the loaded spheres' contact limit; returns
+    dem_search_scales (input_reader.py:142-160)."""
     solv.validate_dem()
     if dem.pos.shape[0] == 0:
         raise ValueError("DEM enabled but input_DEM.txt contains no particles")
-    mass = dem.m.detach().cpu().numpy().astype(float)
-    stiffness = dem.K.detach().cpu().numpy().astype(float)
-    damping = dem.eta.detach().cpu().numpy().astype(float)
-    m_eff = 0.5 * mass.min()
-    limit = solv.dem_dt_safety * min(np.sqrt(m_eff / stiffness.max()),
-                                     m_eff / max(float(damping.max()), 1e-20))
+    limit = dem_contact_dt_limit(solv, dem)
     if solv.dt > limit:
         raise ValueError(f"dt exceeds loaded DEM contact limit {limit:.6g} s")
     return dem_search_scales(dem, boundary)
 
 
+def _load(solv, kinds):
+    folder = input_directory(solv)
+    return [input_parser(folder / f"input_{k}.txt", k, solv.device) for k in kinds]
+
+
 def load_sph_particles(solv):
-    """Read the two or four WCSPH particle files selected by dem_enable
-    (input_reader.py:171-184)."""
+    """The WCSPH files: fluid and wall, plus DEM and DEM wall when
+    dem_enable (input_reader.py:171-184)."""
     solv.validate_sph()
-    directory = input_directory(solv)
-    sph, bnd = (input_parser(directory / f"input_{k}.txt", k, solv.device) for k in ("SPH", "BND"))
+    sph, bnd = _load(solv, ("SPH", "BND"))
     if sph.pos.shape[0] == 0:
         raise ValueError("input_SPH.txt must contain fluid particles")
-    dem = dem_bnd = None
-    if solv.dem_enable:
-        dem, dem_bnd = (input_parser(directory / f"input_{k}.txt", k, solv.device)
-                        for k in ("DEM", "DEMBND"))
-        validate_dem_particles(solv, dem, dem_bnd)
+    if not solv.dem_enable:
+        return sph, bnd, None, None
+    dem, dem_bnd = _load(solv, ("DEM", "DEMBND"))
+    validate_dem_particles(solv, dem, dem_bnd)
     return sph, bnd, dem, dem_bnd
 
 
 def validate_eisph_particles(solv, fluid, boundary) -> None:
-    """A fixed planar EISPH cloud and its mirror map (input_reader.py:187-208):
+    """A fixed planar cavity cloud and its mirror map (input_reader.py:187-208):
     nonempty sets, mirror rows inside the fluid file, uniform rho = rho0,
     points in the x-z plane, and the explicit predictor's advection and
     viscosity dt limits for the loaded speeds and spacing."""
     solv.validate_eisph()
-    n = fluid.pos.shape[0]
-    if n == 0 or boundary.pos.shape[0] == 0:
+    n_fluid = fluid.pos.shape[0]
+    if n_fluid == 0 or boundary.pos.shape[0] == 0:
         raise ValueError("EISPH needs nonempty fluid and ghost files")
-    if np.any(boundary.mirror.detach().cpu().numpy() >= n):
+    if (_host(boundary.mirror) >= n_fluid).any():
         raise ValueError("EISPHBND mirror is outside the input_EISPH fluid row range")
-    for particle in (fluid, boundary):
-        if not np.allclose(particle.rho.detach().cpu().numpy(), solv.rho0, rtol=1e-6, atol=0):
+    for cloud in (fluid, boundary):
+        if not np.allclose(_host(cloud.rho), solv.rho0, rtol=1e-6, atol=0):
             raise ValueError("EISPH currently requires uniform rho equal to rho0")
-        if not np.allclose(particle.pos.detach().cpu().numpy()[:, 1], 0.0, atol=1e-8):
+        if not np.allclose(_host(cloud.pos)[:, 1], 0.0, atol=1e-8):
             raise ValueError("EISPH points must lie in the x-z plane (y=0)")
-    m = fluid.m.detach().cpu().numpy()
-    rho = fluid.rho.detach().cpu().numpy()
-    dx = float(np.sqrt(m / rho).min())
-    speed = max(float(np.linalg.norm(fluid.vel.detach().cpu().numpy(), axis=1).max()),
-                float(np.linalg.norm(boundary.vel_bc.detach().cpu().numpy(), axis=1).max()))
-    if speed > 0 and solv.dt > 0.25 * dx / speed:
+    # planar mass m = rho dx^2: the smallest equivalent spacing
+    spacing = float(np.sqrt(_host(fluid.m) / _host(fluid.rho)).min())
+    speeds = [np.linalg.norm(_host(v), axis=1).max() for v in (fluid.vel, boundary.vel_bc)]
+    speed = float(max(speeds))
+    if speed > 0 and solv.dt > 0.25 * spacing / speed:
         raise ValueError("dt exceeds the loaded EISPH initial advection limit")
-    if solv.nu > 0 and solv.dt > 0.125 * dx * dx / solv.nu:
+    if solv.nu > 0 and solv.dt > 0.125 * spacing ** 2 / solv.nu:
         raise ValueError("dt exceeds the loaded EISPH viscosity limit")
 
 
 def load_eisph_particles(solv):
-    """input_EISPH.txt and input_EISPHBND.txt, validated, with the initial
-    ghost velocity 2 vel_bc - vel[mirror] in vel and vel_star
-    (input_reader.py:211-220)."""
-    directory = input_directory(solv)
-    fluid, boundary = (input_parser(directory / f"input_{k}.txt", k, solv.device)
-                       for k in ("EISPH", "EISPHBND"))
-    validate_eisph_particles(solv, fluid, boundary)
-    mirror = boundary.mirror.long()
-    boundary.vel = 2.0 * boundary.vel_bc - fluid.vel[mirror]
-    boundary.vel_star = boundary.vel.clone()
-    return fluid, boundary
+    """The cavity files, validated, with each ghost's initial velocity
+    2 vel_bc - vel[mirror] in vel and vel_star (input_reader.py:211-220)."""
+    fluid, ghosts = _load(solv, ("EISPH", "EISPHBND"))
+    validate_eisph_particles(solv, fluid, ghosts)
+    ghosts.vel = 2.0 * ghosts.vel_bc - fluid.vel[ghosts.mirror.long()]
+    ghosts.vel_star = ghosts.vel.clone()
+    return fluid, ghosts
