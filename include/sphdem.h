@@ -321,11 +321,15 @@ int sphdem_profile_read(double *ms, int *counts, int n);
  *   xcd_remap[pass]  1: hand consecutive subject tiles to one XCD, so that
  *                    neighbouring tiles share that XCD's L2 (default 0)
  *   por_x_tile       log2 of the buckets along x that one porosity stream
- *                    sub-group may span: 2..6 (default 4) */
+ *                    sub-group may span: 2..6 (default 4)
+ *   dem_beside       1: in the fused coupled step the DEM interpolation runs
+ *                    on the side stream beside the porosity stream kernel;
+ *                    0: after it, on the caller's stream (default 1) */
 typedef struct {
     int block[SPHDEM_NUM_PASSES];
     int xcd_remap[SPHDEM_NUM_PASSES];
     int por_x_tile;
+    int dem_beside;
 } sphdem_tuning_t;
 int sphdem_tuning_get(sphdem_tuning_t *t);
 int sphdem_tuning_set(const sphdem_tuning_t *t);

@@ -89,7 +89,7 @@ def test_judged_call_keeps_the_plain_result_and_rejection_reverts():
     solv, structs = wet()
     net = MPNN(DEV, seed=2)
     # keep: 2 surrogate calls, then a judged call == 2 surrogate calls + 10 plain steps
-    a = _rollout(solv, structs, net, judge_every=3, density_tol=1e9)
+    a = _rollout(solv, structs, net, judge_every=3, density_tol=1e9, velocity_tol=1e9)
     a.run(3)
     b = _rollout(solv, structs, net, judge_every=10**6)
     b.run(2)

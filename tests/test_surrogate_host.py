@@ -11,15 +11,17 @@ class Stub(SurrogateRollout):
     """State = list of call kinds applied; the 'surrogate' goes bad at the
     calls listed in `bad` (any state containing a bad call is unhealthy)."""
 
-    def __init__(self, judge_every, fallback_calls, bad=()):
-        self.judge_every, self.density_tol = judge_every, 0.01
+    def __init__(self, judge_every, fallback_calls, bad=(), slow=()):
+        self.judge_every, self.density_tol, self.velocity_tol = judge_every, 0.01, 0.02
         self.fallback_calls = fallback_calls
         self.stats = RolloutStats()
         self._fallback_left = 0
         self._ckpt = None
         self.trace = []
         self.bad = set(bad)
+        self.slow = set(slow)       # calls whose velocity error fails the judge
         self.flags = False
+        self.insane_end = False
 
     def _state(self):
         return list(self.trace)
@@ -43,13 +45,27 @@ class Stub(SurrogateRollout):
     def healthy(self):
         return "S!" not in self.trace
 
+    def health(self):
+        return {"ok": not self.insane_end or "S" not in self.trace}
+
+    def sane(self, h):
+        return h["ok"]
+
+    def _save_start(self):
+        return []
+
+    def _restart_plain(self):
+        self.trace = []
+
     def _dem_flags(self):
         return self.flags
 
     def judged_call(self):
         ok = self.healthy()
+        n = len(self.trace) + 1
         self.trace.append("P")
-        return {"density_err": 0.001 if ok else float("inf"), "velocity_err": 0.0}
+        return {"density_err": 0.001, "velocity_err": 0.5 if n in self.slow else 0.001,
+                "state_ok": ok}
 
 
 def test_clean_run_keeps_every_surrogate_call_and_judges_on_cadence():
@@ -79,3 +95,21 @@ def test_calls_after_the_last_judge_are_validated():
     r.finish()
     assert r.trace == ["S"] * 4 + ["P"] + ["S"] * 4 + ["P"] + ["P"] * 3
     assert r.stats.rejected == 1 and r.stats.surrogate_calls == 8
+
+
+def test_velocity_error_alone_rejects_a_judged_call():
+    r = Stub(judge_every=5, fallback_calls=2, slow={5})
+    r.run(10)
+    r.finish()
+    # the judge at call 5 fails on velocity: calls 1-5 redone plain, 2 fallback
+    assert r.trace == ["P"] * 5 + ["P"] * 2 + ["S"] * 2 + ["P"]
+    assert r.stats.rejected == 1 and r.stats.surrogate_calls == 2
+
+
+def test_an_insane_end_state_is_redone_from_the_start_with_the_plain_step():
+    r = Stub(judge_every=5, fallback_calls=2)
+    r.insane_end = True
+    r.run(8)
+    r.finish()
+    assert r.trace == ["P"] * 8
+    assert r.stats.restarted_plain and r.stats.surrogate_calls == 0

@@ -98,7 +98,7 @@ class EisphNbr(ctypes.Structure):
 
 class TuningStruct(ctypes.Structure):
     _fields_ = [("block", c_int * NUM_PASSES), ("xcd_remap", c_int * NUM_PASSES),
-                ("por_x_tile", c_int)]
+                ("por_x_tile", c_int), ("dem_beside", c_int)]
 
 
 _SIGNATURES = {
@@ -415,7 +415,7 @@ def get_tuning() -> dict:
     call("sphdem_tuning_get", t)
     return {"block": {n: int(t.block[i]) for i, n in enumerate(PASS_NAMES)},
             "xcd_remap": {n: int(t.xcd_remap[i]) for i, n in enumerate(PASS_NAMES)},
-            "por_x_tile": int(t.por_x_tile)}
+            "por_x_tile": int(t.por_x_tile), "dem_beside": int(t.dem_beside)}
 
 
 def set_tuning(cfg: dict) -> None:
@@ -426,6 +426,7 @@ def set_tuning(cfg: dict) -> None:
         t.block[i] = int(cfg.get("block", {}).get(n, cur["block"][n]))
         t.xcd_remap[i] = int(cfg.get("xcd_remap", {}).get(n, cur["xcd_remap"][n]))
     t.por_x_tile = int(cfg.get("por_x_tile", cur["por_x_tile"]))
+    t.dem_beside = int(cfg.get("dem_beside", cur["dem_beside"]))
     call("sphdem_tuning_set", t)
 
 

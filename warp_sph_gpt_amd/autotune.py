@@ -22,6 +22,20 @@ REMAPS = (0, 1)
 X_TILES = (3, 4, 5)
 
 
+def _time_whole_steps(step_fn, steps: int) -> float:
+    """Milliseconds of `steps` steps on the current stream (every pass, both
+    streams: the side stream joins back before a step ends)."""
+    step_fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        step_fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b)
+
+
 def _time_steps(step_fn, steps: int) -> dict:
     step_fn()                      # settle the new shapes (first-launch costs)
     torch.cuda.synchronize()
@@ -64,7 +78,19 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
             xt_ms[xt] = _time_steps(step_fn, steps).get("porosity", 0.0)
         cfg["por_x_tile"] = min(xt_ms, key=xt_ms.get)
         _lib.set_tuning({"por_x_tile": cfg["por_x_tile"]})
+    # where the DEM interpolation runs changes only the overlap: whole steps
+    beside_ms = {}
+    if "porosity" in passes:
+        for b in (1, 0):
+            _lib.set_tuning({"dem_beside": b})
+            beside_ms[b] = _time_whole_steps(step_fn, steps)
+        cfg["dem_beside"] = min(beside_ms, key=beside_ms.get)
+        _lib.set_tuning({"dem_beside": cfg["dem_beside"]})
+        if log:
+            log(f"[autotune] step ms with the DEM interpolation beside / after the porosity: "
+                f"{beside_ms[1] / steps:.3f} / {beside_ms[0] / steps:.3f}")
     out = {"tuning": cfg, "ms": table, "por_x_tile_ms": {str(k): v for k, v in xt_ms.items()},
+           "dem_beside_ms": {str(k): v for k, v in beside_ms.items()},
            "steps_per_candidate": steps}
     if log:
         log(f"[autotune] chosen {json.dumps(cfg)}")
@@ -80,6 +106,8 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
 
 
 def reset() -> None:
-    """Back to the library defaults (256 threads, no remap, x tile 4)."""
+    """Back to the library defaults (256 threads, no remap, x tile 4, the
+    DEM interpolation beside the porosity)."""
     _lib.set_tuning({"block": {p: 256 for p in _lib.PASS_NAMES},
-                     "xcd_remap": {p: 0 for p in _lib.PASS_NAMES}, "por_x_tile": 4})
+                     "xcd_remap": {p: 0 for p in _lib.PASS_NAMES}, "por_x_tile": 4,
+                     "dem_beside": 1})

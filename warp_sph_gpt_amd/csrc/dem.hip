@@ -699,7 +699,7 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         int *near = dem_near_flags(p, D, gd, w, csup);
         // with the tiled porosity the DEM interpolation needs only the mark's
         // flags: it runs on the side stream beside the porosity stream kernel
-        const bool beside = side && p->h_uniform > 0.0 && gs->n > 0;
+        const bool beside = side && p->h_uniform > 0.0 && gs->n > 0 && tuning().dem_beside;
         rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near,
                          beside ? side->marked : nullptr);
         if (rc) {

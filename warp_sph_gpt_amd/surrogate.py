@@ -506,6 +506,7 @@ class RolloutStats:
     velocity_err: list = field(default_factory=list)
     unhealthy: int = 0
     flag_bits: int = 0
+    restarted_plain: bool = False
 
 
 class SurrogateRollout:
@@ -514,11 +515,12 @@ class SurrogateRollout:
 
     def __init__(self, solv, P, B, D, W, net: MPNN, judge_every: int = 50,
                  density_tol: float = 0.01, fallback_calls: int = 50,
-                 fast_math: bool = True):
+                 fast_math: bool = True, velocity_tol: float = 0.02):
         from .simulation import solver_params
         self.eng = CoupledEngine(solv, P, B, D, W, fast_math=fast_math)
         self.net, self.solv = net, solv
         self.judge_every, self.density_tol = int(judge_every), float(density_tol)
+        self.velocity_tol = float(velocity_tol)
         self.health_tol = 0.01
         wall = self.eng.B.pos
         self._box_lo = wall.min(0).values - float(solv.support)
@@ -617,8 +619,12 @@ class SurrogateRollout:
                   self._gj.struct(), e.gb.struct(), st)
 
     def judged_call(self) -> dict:
-        """Surrogate and plain from the same state; the plain result is kept."""
+        """Surrogate and plain from the same state S; the plain result is
+        kept.  Returns the surrogate's errors against it and whether S (the
+        state the unjudged calls since the last checkpoint produced) and the
+        plain result are sound (see sound())."""
         e = self.eng
+        s_ok = self.sound(self.health())
         saved = self._state()
         k = e.step_index
         self.surrogate_call()
@@ -633,12 +639,13 @@ class SurrogateRollout:
         self._density(e.P.pos, rho_p)
         rho0 = float(self.solv.rho0)
         d_err = float(torch.sqrt((((rho_s - rho_p) / rho0).double() ** 2).mean()))
-        if self._dem_flags() or not self.healthy():
-            d_err = float("inf")
         dv = (v_s - e.P.vel).double()
         v_err = float(torch.sqrt((dv ** 2).sum(1).mean())
                       / torch.sqrt((e.P.vel.double() ** 2).sum(1).mean()).clamp_min(1e-30))
-        return {"density_err": d_err, "velocity_err": v_err}
+        ok = s_ok and not self._dem_flags() and self.sound(self.health())
+        if not ok:
+            self.stats.unhealthy += 1
+        return {"density_err": d_err, "velocity_err": v_err, "state_ok": ok}
 
     def _dem_flags(self) -> bool:
         """True (and cleared) when a DEM substep overflowed a contact buffer
@@ -650,19 +657,33 @@ class SurrogateRollout:
             self.stats.flag_bits |= v
         return bool(v & 7)
 
+    def sane(self, h: dict) -> bool:
+        """Absolute sanity of a state: finite densities and speeds, every
+        fluid particle inside the tank's box (1 m of head room above it), no
+        speed above a third of the sound speed."""
+        return (math.isfinite(h["density_rms"]) and h["inside"] and math.isfinite(h["vmax"])
+                and h["vmax"] < float(self.solv.c0) / 3.0)
+
+    def sound(self, h: dict) -> bool:
+        """sane(), and the Shepard density's deviation from rho0 no worse
+        than the last checkpoint's (a state the plain step produced and
+        the judge accepted) by more than health_tol in RMS and 5 % at most
+        (with 15 % allowed in any case: the plain step's own splash reaches
+        ~20 % where the front hits the far wall)."""
+        ref = self._ref_health
+        return (self.sane(h) and h["density_rms"] <= ref["density_rms"] + self.health_tol
+                and h["density_max"] <= max(0.15, ref["density_max"] + 0.05))
+
     def healthy(self) -> bool:
-        """The current state is finite, every fluid particle is inside the
-        tank's box (1 m of head room above it), no speed exceeds a third of
-        the sound speed, and the Shepard density at the current positions
-        deviates from rho0 by less than health_tol in RMS and 15 % at most
-        (the plain path keeps about 0.4 % and 8 %)."""
-        ok = self.health()["ok"]
+        """The current state is sound() against the last checkpoint."""
+        ok = self.sound(self.health())
         if not ok:
             self.stats.unhealthy += 1
         return ok
 
     def health(self, P=None) -> dict:
-        """The quantities healthy() tests, for this rollout's fluid or for
+        """Density deviation (RMS and maximum, Shepard density at the
+        positions), box and speed figures of this rollout's fluid or of
         another state of the same scene (e.g. the plain run's)."""
         P = self.eng.P if P is None else P
         rho = P.rho.clone()
@@ -673,14 +694,14 @@ class SurrogateRollout:
         pos = P.pos
         inside = bool(((pos >= self._box_lo) & (pos <= self._box_hi)).all())
         vmax = float(P.vel.abs().max())
-        ok = (math.isfinite(dev) and dev < self.health_tol and dmax < 0.15 and inside
-              and math.isfinite(vmax) and vmax < float(self.solv.c0) / 3.0)
-        return {"density_rms": dev, "density_max": dmax, "inside": inside, "vmax": vmax,
-                "ok": ok}
+        h = {"density_rms": dev, "density_max": dmax, "inside": inside, "vmax": vmax}
+        h["ok"] = self.sane(h)
+        return h
 
     # ---- loop
     def _checkpoint(self) -> None:
         self._ckpt = (self._state(), self.eng.step_index, self.stats.calls)
+        self._ref_health = self.health()
         self._sur_since = 0
 
     def _restore(self) -> int:
@@ -691,12 +712,16 @@ class SurrogateRollout:
         return self.stats.calls - calls
 
     def run(self, n_calls: int) -> RolloutStats:
-        """n_calls calls of ten substeps.  A judged call whose density error
-        exceeds density_tol (or is not finite) reverts the state to the last
-        judged call's and redoes the calls since then with the plain step,
-        then keeps to the plain step for fallback_calls more calls."""
+        """n_calls calls of ten substeps.  A judged call is accepted when
+        its density error is within density_tol, its velocity error within
+        velocity_tol and the state it started from and its plain result are
+        sound; otherwise the state goes back to the last checkpoint, the
+        calls since then are redone with the plain step, and the plain step
+        runs fallback_calls more calls.  Each judged call ends in a new
+        checkpoint (its plain result)."""
         st = self.stats
         if getattr(self, "_ckpt", None) is None:
+            self._start = self._save_start()
             self._checkpoint()
         for _ in range(int(n_calls)):
             st.calls += 1
@@ -710,7 +735,8 @@ class SurrogateRollout:
                 st.plain_calls += 1
                 st.density_err.append(r["density_err"])
                 st.velocity_err.append(r["velocity_err"])
-                if not r["density_err"] <= self.density_tol:
+                if not (r["state_ok"] and r["density_err"] <= self.density_tol
+                        and r["velocity_err"] <= self.velocity_tol):
                     st.rejected += 1
                     redo = self._restore()
                     for _ in range(redo):
@@ -727,10 +753,12 @@ class SurrogateRollout:
 
     def finish(self) -> None:
         """Validate the calls since the last judged one: if a DEM substep
-        flagged them or the state is not healthy, revert and redo them with
-        the plain step."""
+        flagged them or the state is not sound, revert and redo them with
+        the plain step.  If the end state is still not sane (a checkpoint
+        the plain step itself cannot carry on from), the whole run is redone
+        with the plain step from its start."""
+        st = self.stats
         if self._sur_since and (self._dem_flags() or not self.healthy()):
-            st = self.stats
             st.rejected += 1
             redo = self._restore()
             for _ in range(redo):
@@ -738,6 +766,23 @@ class SurrogateRollout:
             st.plain_calls += redo
             st.surrogate_calls -= self._sur_since
             self._checkpoint()
+        if not self.sane(self.health()):
+            self._restart_plain()
+            st.restarted_plain = True
+            for _ in range(st.calls):
+                self.plain_call()
+            st.plain_calls += st.calls
+            st.surrogate_calls = 0
+            self._checkpoint()
+
+    def _save_start(self):
+        return self._state(), self.eng.step_index
+
+    def _restart_plain(self) -> None:
+        saved, k = self._start
+        for s, name, t in saved:
+            getattr(s, name).copy_(t)
+        self.eng.step_index = k
 
 
 # ---------------------------------------------------------------- command line
@@ -779,7 +824,8 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
 
 
 def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
-               density_tol: float = 0.01, device="cuda:0", log=None) -> dict:
+               density_tol: float = 0.01, device="cuda:0", log=None,
+               velocity_tol: float = 0.02) -> dict:
     """C1 dam break (100,000 fluid, 200 DEM): plain GPU path and surrogate
     rollout over the same simulated time, both replaying hipGraphs."""
     from .workloads import c1
@@ -797,7 +843,7 @@ def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
     t_plain = time.perf_counter() - t0
     plain.check()
     ro = SurrogateRollout(wl.solv, *[s.clone() for s in structs], net, judge_every=judge_every,
-                          density_tol=density_tol)
+                          density_tol=density_tol, velocity_tol=velocity_tol)
     ro.plain_call()                                 # first call plain (eager warm-up)
     ro.capture()
     torch.cuda.synchronize()
@@ -816,6 +862,7 @@ def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
            "surrogate_sim_per_wall": sim_sur, "speedup": sim_sur / sim_plain,
            "surrogate_calls": st.surrogate_calls, "plain_calls": st.plain_calls,
            "judged": st.judged, "rejected": st.rejected, "unhealthy": st.unhealthy,
+           "restarted_plain": st.restarted_plain,
            "dem_flag_bits": st.flag_bits,
            "density_err_max": max(st.density_err) if st.density_err else None,
            "density_err_mean": float(np.mean(st.density_err)) if st.density_err else None,
