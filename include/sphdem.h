@@ -124,7 +124,7 @@ typedef struct {
     int *scan_tmp;                  /* [sphdem_scan_scratch(n_dem + 1)] */
     int *flags;                     /* [4] bit 0: DEM CSR overflow */
     int *fl_flag;                   /* [nf] near-DEM flags */
-    int *fl_list;                   /* [2*nf] porosity targets, flag offsets */
+    int *fl_list;                   /* [2*nf] see fl_sum */
     int *fl_count;                  /* [1] number of porosity targets */
     int *fl_scan;                   /* [sphdem_scan_scratch(nf)] */
     /* [cap_d] or NULL: per-DEM flags, all zero between steps.  In the fused
@@ -148,6 +148,14 @@ typedef struct {
      * wall sets of 4M particles or more, where it pays.  NULL: every wall
      * particle runs its fluid query. */
     void *fl_box;
+    /* [cap_f] REAL or NULL: with fast_math on FP32 and a uniform DEM h, the
+     * porosity's fluid sums are evaluated symmetrically (a pair of near-DEM
+     * subjects once, for both sums: the same terms, other rounding), with
+     * each subject's own share here and the mirrored terms in fl_list, read
+     * as cap_f uint64 fixed-point accumulators that must be zero between
+     * steps (zero-filled once at allocation; the step clears what it used).
+     * NULL: every subject sums its own terms. */
+    void *fl_sum;
     int cap_f, cap_b, cap_d;
 } sphdem_work_t;
 

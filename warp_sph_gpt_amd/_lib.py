@@ -70,7 +70,8 @@ class GridStruct(ctypes.Structure):
 class WorkStruct(ctypes.Structure):
     _fields_ = _ptr_fields(("fl_rho", "fl_pres", "fl_V", "fl_vel", "fl_pgf", "wl_rho",
                             "wl_pres", "wl_vel", "scan_tmp", "flags", "fl_flag", "fl_list",
-                            "fl_count", "fl_scan", "dem_near", "wl_static", "fl_box")) + [
+                            "fl_count", "fl_scan", "dem_near", "wl_static", "fl_box",
+                            "fl_sum")) + [
         ("cap_f", c_int), ("cap_b", c_int), ("cap_d", c_int)]
 
 

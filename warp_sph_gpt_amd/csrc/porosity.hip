@@ -26,6 +26,14 @@
 // subject of the group are dropped before them (mfma_blocks and
 // queue_range_mfma below), which changes the rounding of the sums, not
 // their terms.
+// FAST, FP32, symmetric (when the work struct brings fl_sum): W(r_ij) is the
+// same for i's sum and j's, so a pair of flagged subjects in different waves
+// is evaluated once, by the lower wave: it adds V_j W to i's sum and V_i W to
+// j's, and the higher wave drops i from its stream.  The mirrored terms of a
+// candidate are summed over the wave's subjects and added to j's fixed-point
+// accumulator with a 64-bit integer atomic (order-free, so the result does
+// not depend on which wave adds first); a last pass adds the subject's own
+// sum and finishes the porosity.  Same terms, other rounding.
 #include "common.hpp"
 
 #include <climits>
@@ -255,6 +263,91 @@ __device__ __forceinline__ void mfma_blocks(f32x4 (&acc)[4], const float (&a)[4]
     }
 }
 
+// ---- symmetric FP32 fast path
+// The wave's slot range, the flags, the fixed-point accumulators and their
+// scale (units per unit of the unscaled sum; a power of two).
+struct SymCtx {
+    const int *flag;
+    unsigned long long *col;
+    float scale;
+    int w0, w1;
+};
+
+// v >= 0 in fixed point: x = v scale (exact), split at 2^32 exactly (the
+// high part truncated, the low part an exact float difference by Sterbenz).
+__device__ __forceinline__ unsigned long long to_fixed(float v, float scale) {
+    const float x = v * scale;
+    const float hi = truncf(x * 2.3283064365386963e-10f);
+    const unsigned lo = (unsigned)(x - hi * 4294967296.0f);
+    return ((unsigned long long)(unsigned)hi << 32) | lo;
+}
+
+// The sum of v over lanes l%16 + {0, 16, 32, 48}, in every lane: the
+// halves' sums by a 32-lane swap, then the rows' by a 16-lane swap
+// (v_permlane32_swap, v_permlane16_swap: no LDS round trip).
+__device__ __forceinline__ float quarter_sum(float v) {
+    const auto h = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
+                                                    false, false);
+    const float s = __uint_as_float(h[0]) + __uint_as_float(h[1]);
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(s), __float_as_uint(s),
+                                                    false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// The symmetric queue's blocks: as mfma_blocks, and each lane also sums
+// V_i W over its 16 (subject, candidate) pairs for its candidate (column
+// l%16); the four quarters' sums are added and lanes 0..15 add the total to
+// the candidate's accumulator.  vsub: V of the wave's subjects by lane (0
+// outside the group); sid: the candidates' sorted slots (-1 for padding).
+// (Issuing the next block's products before this block's terms, with one
+// set of results or two, measured slower.)
+__device__ __forceinline__ void mfma_blocks_sym(f32x4 (&acc)[4], const float (&a)[4],
+                                                const f32x4 (&c)[4], const V4<float> *buf,
+                                                const float *vbuf, const int *sid, int nsub,
+                                                int lane, float k1, float k2, const float *vsub,
+                                                const SymCtx &sym) {
+    const int col = lane & 15, comp = lane >> 4;
+    for (int sc = 0; sc < nsub; ++sc) {
+        const float bv = reinterpret_cast<const float *>(buf + 16 * sc + col)[comp];
+        const float vj = vbuf[16 * sc + col];
+        f32x4 d[4];
+#pragma unroll
+        for (int T = 0; T < 4; ++T)
+            d[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[T], bv, c[T], 0, 0, 0);
+        float cs = 0.0f;
+#pragma unroll
+        for (int T = 0; T < 4; ++T) {
+            // V of subjects 16T + 4(l/16) + 0..3: one broadcast read per quarter
+            const f32x4 vv = *reinterpret_cast<const f32x4 *>(vsub + 16 * T + 4 * comp);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float r = fast_sqrt(fabsf(d[T][t]));
+                const float u = clamp01(fmaf(k1, r, 1.0f));
+                const float pq = fmaf(k2, r, 1.0f);
+                float w = u * u;
+                w = w * w;
+                w = w * pq;
+                acc[T][t] = fmaf(vj, w, acc[T][t]);
+                cs = fmaf(vv[t], w, cs);
+            }
+        }
+        cs = quarter_sum(cs);
+        if (lane < 16) {
+            const int j = sid[16 * sc + col];
+            if (j >= 0 && cs > 0.0f) atomicAdd(sym.col + j, to_fixed(cs, sym.scale));
+        }
+    }
+}
+
+// Symmetric queue of a wave (SYM): candidates, V, sorted slots and fill.
+struct SymQueue {
+    V4<float> *buf;
+    float *vbuf;
+    int *sid;
+    float *vsub;   // [64] V of the wave's subjects by lane (0 outside the group)
+    int fill;
+};
+
 // FAST, FP32, culled: a candidate farther than the support from the box of
 // the group's subjects adds 0 to every subject's sum, so it is dropped as it
 // is loaded; the others queue in the wave's LDS slot (128 entries, `fill`
@@ -263,24 +356,29 @@ __device__ __forceinline__ void mfma_blocks(f32x4 (&acc)[4], const float (&a)[4]
 // measured on the relative coordinates the queue stores.  The test uses
 // r2 = support^2 (1 + 1e-4) against a distance rounded to ~1e-7 m, so no
 // candidate within the support is dropped.
+template <bool SYM>
 __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&a)[4],
                                                  const f32x4 (&c)[4], int &fill, int a0, int b0,
                                                  const V4<float> *pos, const float *Vs,
                                                  V4<float> *buf, float *vbuf, int lane,
                                                  const V3<float> &gc, const V3<float> &he,
-                                                 float r2, float k1, float k2) {
+                                                 float r2, float k1, float k2, SymQueue &sq,
+                                                 const SymCtx &sym) {
     if (b0 <= a0) return;
     // slots past the range read its last slot again and are never kept
-    auto load = [&](int s, float &v) {
+    auto load = [&](int s, float &v, int &f) {
         const int t = min(s, b0 - 1);
         v = Vs[t];
+        f = SYM ? sym.flag[t] : 0;
         return pos[t];
     };
     float vcur;
-    V4<float> cur = load(a0 + lane, vcur);
+    int fcur;
+    V4<float> cur = load(a0 + lane, vcur, fcur);
     for (int s0 = a0; s0 < b0; s0 += 64) {
         float vnxt;
-        const V4<float> nxt = load(s0 + 64 + lane, vnxt);
+        int fnxt;
+        const V4<float> nxt = load(s0 + 64 + lane, vnxt, fnxt);
         V4<float> q;
         q.x = cur.x - gc.x;
         q.y = cur.y - gc.y;
@@ -288,17 +386,48 @@ __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&
         const float ex = fmaxf(fabsf(q.x) - he.x, 0.0f);
         const float ey = fmaxf(fabsf(q.y) - he.y, 0.0f);
         const float ez = fmaxf(fabsf(q.z) - he.z, 0.0f);
-        const bool keep = s0 + lane < b0 && fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= r2;
+        const int s = s0 + lane;
+        const bool near = s < b0 && fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= r2;
+        // SYM: a flagged candidate of a lower wave is that wave's pair; one
+        // of a higher wave goes to the symmetric queue
+        const bool higher = SYM && fcur != 0 && s >= sym.w1;
+        const bool keep = near && !(SYM && fcur != 0 && (s < sym.w0 || s >= sym.w1));
+        q.w = fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x));
         const unsigned long long mask = __ballot(keep);
         if (keep) {
             const int at = fill + (int)__builtin_amdgcn_mbcnt_hi(
                                       (unsigned)(mask >> 32),
                                       __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-            q.w = fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x));
             buf[at] = q;
             vbuf[at] = vcur;
         }
         fill = uniform_i(fill + __popcll(mask));
+        if constexpr (SYM) {
+            const bool keep_s = near && higher;
+            const unsigned long long ms = __ballot(keep_s);
+            if (keep_s) {
+                const int at = sq.fill + (int)__builtin_amdgcn_mbcnt_hi(
+                                             (unsigned)(ms >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((unsigned)ms, 0u));
+                sq.buf[at] = q;
+                sq.vbuf[at] = vcur;
+                sq.sid[at] = s;
+            }
+            sq.fill = uniform_i(sq.fill + __popcll(ms));
+            if (sq.fill >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                mfma_blocks_sym(acc, a, c, sq.buf, sq.vbuf, sq.sid, 4, lane, k1, k2, sq.vsub,
+                                sym);
+                __builtin_amdgcn_wave_barrier();
+                if (lane < sq.fill - 64) {
+                    sq.buf[lane] = sq.buf[64 + lane];
+                    sq.vbuf[lane] = sq.vbuf[64 + lane];
+                    sq.sid[lane] = sq.sid[64 + lane];
+                }
+                sq.fill -= 64;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
         if (fill >= 64) {
             __builtin_amdgcn_wave_barrier();
             mfma_blocks(acc, a, c, buf, vbuf, 4, lane, k1, k2);
@@ -313,7 +442,27 @@ __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&
         }
         cur = nxt;
         vcur = vnxt;
+        fcur = fnxt;
     }
+}
+
+// The symmetric queue's last entries, padded to a multiple of 16 with
+// candidates 1e15 away (V = 0, no slot).
+__device__ __forceinline__ void flush_queue_sym(f32x4 (&acc)[4], const float (&a)[4],
+                                                const f32x4 (&c)[4], SymQueue &sq, int lane,
+                                                float k1, float k2, const SymCtx &sym) {
+    if (sq.fill == 0) return;
+    const int nsub = (sq.fill + 15) >> 4;
+    if (lane >= sq.fill && lane < 16 * nsub) {
+        const float far = 1.0e15f;
+        sq.buf[lane] = V4<float>{far, far, far, 3.0e30f};
+        sq.vbuf[lane] = 0.0f;
+        sq.sid[lane] = -1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    mfma_blocks_sym(acc, a, c, sq.buf, sq.vbuf, sq.sid, nsub, lane, k1, k2, sq.vsub, sym);
+    __builtin_amdgcn_wave_barrier();
+    sq.fill = 0;
 }
 
 // The queue's last entries, padded to a multiple of 16 with candidates
@@ -360,13 +509,33 @@ __global__ void k_por_mark(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T>
     if (!near) P.porosity[gs.ids[k]] = clampr((T)1.0, p.eps_min, p.eps_max);
 }
 
+// KERNEL_SPHDEM_interaction.py:85-96, 111: the solid fraction of the DEM
+// within 2 h_b of `me` over the fluid sum flt, and the clamped porosity.
+template <typename T>
+__device__ __forceinline__ T porosity_of(const Prm<T> &p, const DemV<T> &D, const GridV<T> &gd,
+                                         const V4<T> &me, T flt, T csup) {
+    T solid = (T)0;
+    query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const int b = gd.ids[s];
+            const V4<T> q = gd.s[s];
+            const T hb = D.h[b];
+            const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+            if (dist < (T)2.0 * hb)
+                solid = solid + D.volume[b] * wendland_w(dist, hb) / (flt + (T)1.0e-20);
+        }
+    });
+    return clampr((T)1.0 - solid, p.eps_min, p.eps_max);
+}
+
 // Stream the union of the balls of the wave's lanes with `in` set; other
 // lanes compare against a zero threshold and add nothing.  Returns false
 // (having done nothing) if that union spans more than dims - 2 buckets of
 // the folded grid along some axis.
-template <typename T, bool FAST>
+template <typename T, bool FAST, bool SYM>
 __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &gs, const T *Vs,
-                             V4<T> *wbuf, T *vbuf, int lane, T csup, T thr2, T h, T inv_h) {
+                             V4<T> *wbuf, T *vbuf, int lane, T csup, T thr2, T h, T inv_h,
+                             SymQueue &sq, T vme, const SymCtx &sym) {
     const T big = std::numeric_limits<T>::max();
     const T bxl = uniform_f(wave_min(in ? me.x : big)), bxh = uniform_f(wave_max(in ? me.x : -big));
     const T byl = uniform_f(wave_min(in ? me.y : big)), byh = uniform_f(wave_max(in ? me.y : -big));
@@ -396,11 +565,16 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
     // FP32 FAST: matrix-core distances (mfma_blocks, fed by queue_range_mfma); set up the
     // group's A rows and C values, accumulate, reduce at the end
     constexpr bool kMfma = FAST && std::is_same<T, float>::value;
+    static_assert(!SYM || kMfma, "the symmetric sums are an FP32 fast-path form");
     const int col = lane & 15, comp = lane >> 4;
     float ma[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     f32x4 mc[4];
     f32x4 acc[4] = {};
     int fill = 0;
+    if constexpr (SYM) {
+        sq.vsub[lane] = in ? (float)vme : 0.0f;
+        __builtin_amdgcn_wave_barrier();
+    }
     if constexpr (kMfma) {
 #pragma unroll
         for (int Tt = 0; Tt < 4; ++Tt) {
@@ -445,12 +619,12 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
                 const int b1 = __builtin_amdgcn_readlane(rb1, r);
                 const int a2 = __builtin_amdgcn_readlane(ra2, r);
                 const int b2 = __builtin_amdgcn_readlane(rb2, r);
-                queue_range_mfma(acc, ma, mc, fill, a1, b1, (const V4<float> *)gs.s,
-                                 (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
-                                 hef, (float)r2c, (float)k1, (float)k2);
-                queue_range_mfma(acc, ma, mc, fill, a2, b2, (const V4<float> *)gs.s,
-                                 (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane, gcf,
-                                 hef, (float)r2c, (float)k1, (float)k2);
+                queue_range_mfma<SYM>(acc, ma, mc, fill, a1, b1, (const V4<float> *)gs.s,
+                                      (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane,
+                                      gcf, hef, (float)r2c, (float)k1, (float)k2, sq, sym);
+                queue_range_mfma<SYM>(acc, ma, mc, fill, a2, b2, (const V4<float> *)gs.s,
+                                      (const float *)Vs, (V4<float> *)wbuf, (float *)vbuf, lane,
+                                      gcf, hef, (float)r2c, (float)k1, (float)k2, sq, sym);
             }
         }
     } else {
@@ -486,6 +660,7 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
     if constexpr (kMfma) {
         flush_queue_mfma(acc, ma, mc, fill, (V4<float> *)wbuf, (float *)vbuf, lane, (float)k1,
                          (float)k2);
+        if constexpr (SYM) flush_queue_sym(acc, ma, mc, sq, lane, (float)k1, (float)k2, sym);
         // subject 16T + 4(l/16) + t: add its 16 column partial sums, lanes
         // l%16 = 0..15 of each quarter (fixed xor tree), then hand the total
         // to the subject's own lane through the wave's LDS slot
@@ -521,24 +696,38 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
 // still too wide (folded buckets), in sub-groups of 2^xt buckets along x; a
 // sub-group that still does not fit (grids narrower than the query) walks
 // its own query per lane.
-template <typename T, bool FAST>
+template <typename T, bool FAST, bool SYM>
 __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
                                                     GridV<T> gd, const T *Vs, const int *flag,
-                                                    T csup, T hu, T thr2, int xt, int remap) {
-    // per wave: 64 staged candidates, 128 for the FP32 matrix-core queue
+                                                    T csup, T hu, T thr2, int xt, int remap,
+                                                    T *row_sum, unsigned long long *col_sum,
+                                                    float col_scale) {
+    // per wave: 64 staged candidates, 128 for the FP32 matrix-core queue;
+    // SYM: a second 128-entry queue for the pairs this wave evaluates for both
     constexpr int kSlot = (FAST && std::is_same<T, float>::value) ? 128 : 64;
+    constexpr int kSymSlot = SYM ? 128 : 1;
     __shared__ V4<T> stage[4 * kSlot];
     __shared__ T vstage[4 * kSlot];
+    __shared__ V4<float> sstage[4 * kSymSlot];
+    __shared__ float svstage[4 * kSymSlot];
+    __shared__ int sidstage[4 * kSymSlot];
+    __shared__ __align__(16) float vsubstage[4 * (SYM ? 64 : 1)];
     const int k = subject_id(remap);
     const int lane = threadIdx.x & 63;
-    V4<T> *wbuf = stage + (threadIdx.x >> 6) * kSlot;
-    T *vbuf = vstage + (threadIdx.x >> 6) * kSlot;
+    const int wv = threadIdx.x >> 6;
+    V4<T> *wbuf = stage + wv * kSlot;
+    T *vbuf = vstage + wv * kSlot;
+    SymQueue sq{sstage + wv * kSymSlot, svstage + wv * kSymSlot, sidstage + wv * kSymSlot,
+                vsubstage + wv * (SYM ? 64 : 1), 0};
+    const int w0 = k & ~63;
+    const SymCtx sym{flag, col_sum, col_scale, w0, w0 + 64};
     const bool active = k < gs.n && flag[k] != 0;
     const unsigned long long live = __ballot(active);
     if (live == 0ull) return;
-    const int lead = (int)(k & ~63) + __ffsll((long long)live) - 1;
+    const int lead = w0 + __ffsll((long long)live) - 1;
     const int ks = active ? k : lead;
     const V4<T> me = gs.s[ks];
+    const T vme = SYM ? Vs[ks] : (T)0;
     const int ik = inst_of(gs, ks);
     gd = at_inst(gd, ik);
     const T h = hu, inv_h = (T)1.0 / hu, inv = gs.inv;
@@ -552,19 +741,34 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
             __ballot(active && ky == ly && kz == lz && ik == li) & todo;
         const bool in = (grp >> lane) & 1ull;
         const GridV<T> gg = at_inst(gs, uniform_i(li));
-        if (!stream_group<T, FAST>(flt, me, in, gg, Vs, wbuf, vbuf, lane, csup, thr2, h,
-                                   inv_h)) {
+        if (!stream_group<T, FAST, SYM>(flt, me, in, gg, Vs, wbuf, vbuf, lane, csup, thr2, h,
+                                        inv_h, sq, vme, sym)) {
             unsigned long long rest = grp;
             while (rest) {
                 const int l2 = __ffsll((long long)rest) - 1;
                 const int lx = __shfl(kx, l2, 64);
                 const unsigned long long sub = __ballot(kx == lx) & rest;
                 const bool in2 = (sub >> lane) & 1ull;
-                if (!stream_group<T, FAST>(flt, me, in2, gg, Vs, wbuf, vbuf, lane, csup, thr2,
-                                           h, inv_h) && in2) {
+                if (!stream_group<T, FAST, SYM>(flt, me, in2, gg, Vs, wbuf, vbuf, lane, csup,
+                                                thr2, h, inv_h, sq, vme, sym) && in2) {
                     query_runs(at_inst(gs, ik), me.x, me.y, me.z, csup, [&](int lo, int hi) {
                         for (int s = lo; s < hi; ++s) {
                             const V4<T> q = gs.s[s];
+                            if constexpr (SYM) {
+                                // the pair rule of the streamed groups, per pair
+                                const bool f = flag[s] != 0;
+                                if (f && s < sym.w0) continue;
+                                if (f && s >= sym.w1) {
+                                    T w = (T)0;
+                                    add_term<T, FAST>(w, me, q.x, q.y, q.z, (T)1, thr2, h,
+                                                      inv_h);
+                                    flt = fma(Vs[s], w, flt);
+                                    const float cv = (float)(vme * w);
+                                    if (cv > 0.0f)
+                                        atomicAdd(sym.col + s, to_fixed(cv, sym.scale));
+                                    continue;
+                                }
+                            }
                             add_term<T, FAST>(flt, me, q.x, q.y, q.z, Vs[s], thr2, h, inv_h);
                         }
                     });
@@ -575,19 +779,31 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
         todo &= ~grp;
     }
     if (!active) return;
+    if constexpr (SYM) {
+        // the subject's own share; k_por_finish adds the mirrored terms
+        row_sum[k] = flt;
+        return;
+    }
     if (FAST) flt = flt * ((T)21.0 / ((T)16.0 * (T)SPHDEM_PI * hu * hu * hu));
-    T solid = (T)0;
-    query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
-        for (int s = lo; s < hi; ++s) {
-            const int b = gd.ids[s];
-            const V4<T> q = gd.s[s];
-            const T hb = D.h[b];
-            const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
-            if (dist < (T)2.0 * hb)
-                solid = solid + D.volume[b] * wendland_w(dist, hb) / (flt + (T)1.0e-20);
-        }
-    });
-    P.porosity[gs.ids[k]] = clampr((T)1.0 - solid, p.eps_min, p.eps_max);
+    P.porosity[gs.ids[k]] = porosity_of(p, D, gd, me, flt, csup);
+}
+
+// SYM: each flagged subject's sum is its own share plus the mirrored terms
+// the lower waves added (read, then cleared for the next step), then the
+// porosity as k_por_stream finishes it.
+template <typename T>
+__global__ void k_por_finish(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T> gd,
+                             const int *flag, const T *row_sum, unsigned long long *col_sum,
+                             double inv_scale, T csup, T hu, int remap) {
+    const int k = subject_id(remap);
+    if (k >= gs.n || flag[k] == 0) return;
+    const unsigned long long cv = col_sum[k];
+    col_sum[k] = 0ull;
+    T flt = (T)((double)row_sum[k] + (double)cv * inv_scale);
+    flt = flt * ((T)21.0 / ((T)16.0 * (T)SPHDEM_PI * hu * hu * hu));
+    const V4<T> me = gs.s[k];
+    gd = at_inst(gd, inst_of(gs, k));
+    P.porosity[gs.ids[k]] = porosity_of(p, D, gd, me, flt, csup);
 }
 
 // Smallest x with sqrt(x) >= s (IEEE, round to nearest): sqrt(td) < s is
@@ -617,14 +833,30 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
     // the flags are final here: work that needs only them may start
     if (marked) hipEventRecord(marked, st);
     const T thr2 = sqrt_threshold<T>(cs);
-    if (p->fast_math)
-        hipLaunchKernelGGL((k_por_stream<T, true>), dim3(blocks_for(n, bt)), dim3(bt), 0, st,
-                           prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm);
-    else
-        hipLaunchKernelGGL((k_por_stream<T, false>), dim3(blocks_for(n, bt)), dim3(bt), 0, st,
-                           prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm);
+    const bool sym = p->fast_math && std::is_same<T, float>::value && w->fl_sum;
+    if (sym) {
+        // fixed-point units: the unscaled sum of a full ball (1/C) near 2^40
+        const int e = 40 - (int)std::ceil(std::log2(16.0 * SPHDEM_PI * hu * hu * hu / 21.0));
+        const float scale = std::ldexp(1.0f, e);
+        auto *col = reinterpret_cast<unsigned long long *>(w->fl_list);
+        hipLaunchKernelGGL((k_por_stream<T, true, std::is_same<T, float>::value>),
+                           dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p), view<T>(P),
+                           view<T>(D), view<T>(gs), view<T>(gd), (const T *)w->fl_V, w->fl_flag,
+                           cs, (T)hu, thr2, xt, rm, (T *)w->fl_sum, col, scale);
+        hipLaunchKernelGGL(k_por_finish<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
+                           view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), w->fl_flag,
+                           (const T *)w->fl_sum, col, std::ldexp(1.0, -e), cs, (T)hu, rm);
+    } else if (p->fast_math) {
+        hipLaunchKernelGGL((k_por_stream<T, true, false>), dim3(blocks_for(n, bt)), dim3(bt), 0,
+                           st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
+                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm, nullptr,
+                           nullptr, 0.0f);
+    } else {
+        hipLaunchKernelGGL((k_por_stream<T, false, false>), dim3(blocks_for(n, bt)), dim3(bt), 0,
+                           st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
+                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm, nullptr,
+                           nullptr, 0.0f);
+    }
     return check_launch("porosity");
 }
 template int launch_porosity_tiled<float>(const sphdem_params_t *, const sphdem_sph_t *,

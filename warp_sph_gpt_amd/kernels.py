@@ -44,6 +44,9 @@ class Workspace:
             self.fl_vel = torch.zeros((c, 4), dtype=t, device=d)
             self.fl_pgf = torch.zeros((c, 4), dtype=t, device=d)
             self.fl_flag = torch.zeros(c, dtype=torch.int32, device=d)
+            # the symmetric porosity's per-slot own sums and (as c uint64)
+            # fixed-point accumulators of the mirrored terms, zero between steps
+            self.fl_sum = torch.zeros(c, dtype=t, device=d) if t == torch.float32 else None
             self.fl_list = torch.zeros(2 * c, dtype=torch.int32, device=d)
             self.fl_count = torch.zeros(4, dtype=torch.int32, device=d)
             self.fl_scan = torch.zeros(_lib.load().sphdem_scan_scratch(c), dtype=torch.int32,
@@ -82,6 +85,7 @@ class Workspace:
             setattr(w, name, getattr(self, name).data_ptr())
         w.wl_static = self.wl_static.data_ptr() if self.wl_static is not None else None
         w.fl_box = self.fl_box.data_ptr() if self.fl_box is not None else None
+        w.fl_sum = self.fl_sum.data_ptr() if self.fl_sum is not None else None
         w.cap_f, w.cap_b, w.cap_d = self.cap_f, self.cap_b, self.cap_d
         return w
 
