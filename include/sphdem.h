@@ -125,7 +125,7 @@ typedef struct {
     int *flags;                     /* [4] bit 0: DEM CSR overflow */
     int *fl_flag;                   /* [nf] near-DEM flags */
     int *fl_list;                   /* [2*nf] see fl_sum */
-    int *fl_count;                  /* [1] number of porosity targets */
+    int *fl_count;                  /* [4] see fl_chunk */
     int *fl_scan;                   /* [sphdem_scan_scratch(nf)] */
     /* [cap_d] or NULL: per-DEM flags, all zero between steps.  In the fused
      * coupled step the porosity mark sets dem_near[b] = 1 where some fluid
@@ -150,12 +150,18 @@ typedef struct {
     void *fl_box;
     /* [cap_f] REAL or NULL: with fast_math on FP32 and a uniform DEM h, the
      * porosity's fluid sums are evaluated symmetrically (a pair of near-DEM
-     * subjects once, for both sums: the same terms, other rounding), with
-     * each subject's own share here and the mirrored terms in fl_list, read
-     * as cap_f uint64 fixed-point accumulators that must be zero between
-     * steps (zero-filled once at allocation; the step clears what it used).
-     * NULL: every subject sums its own terms. */
+     * subjects once, for both sums: the same terms, other rounding) and
+     * added in fixed point: fl_list is read as cap_f uint64 accumulators
+     * that must be zero between steps (zero-filled once at allocation; the
+     * step clears what it used).  The pass leaves each near-DEM subject's
+     * unscaled sum here.  NULL: every subject sums its own terms. */
     void *fl_sum;
+    /* [7 * cap_f] ints or NULL: scratch of the porosity's work list (the
+     * near-DEM slots of each bucket row, compacted and cut every 64; one
+     * wavefront per entry; counts in fl_count[0..1]), rebuilt every step.
+     * NULL: one wavefront per 64 consecutive slots (the same sums; with the
+     * reference's arithmetic the same bits). */
+    int *fl_chunk;
     int cap_f, cap_b, cap_d;
 } sphdem_work_t;
 
@@ -332,12 +338,19 @@ int sphdem_profile_read(double *ms, int *counts, int n);
  *                    sub-group may span: 2..6 (default 4)
  *   dem_beside       1: in the fused coupled step the DEM interpolation runs
  *                    on the side stream beside the porosity stream kernel;
- *                    0: after it, on the caller's stream (default 1) */
+ *                    0: after it, on the caller's stream (default 1)
+ *   por_parts        with the porosity work list and the symmetric FP32
+ *                    sums: waves per work item, each streaming every
+ *                    por_parts-th bucket row of the item's query: 1..16
+ *                    (default 4).  The terms are the same; their grouping
+ *                    into fixed-point partial sums changes with it, so the
+ *                    sums may differ in the last units of that form. */
 typedef struct {
     int block[SPHDEM_NUM_PASSES];
     int xcd_remap[SPHDEM_NUM_PASSES];
     int por_x_tile;
     int dem_beside;
+    int por_parts;
 } sphdem_tuning_t;
 int sphdem_tuning_get(sphdem_tuning_t *t);
 int sphdem_tuning_set(const sphdem_tuning_t *t);

@@ -71,7 +71,7 @@ class WorkStruct(ctypes.Structure):
     _fields_ = _ptr_fields(("fl_rho", "fl_pres", "fl_V", "fl_vel", "fl_pgf", "wl_rho",
                             "wl_pres", "wl_vel", "scan_tmp", "flags", "fl_flag", "fl_list",
                             "fl_count", "fl_scan", "dem_near", "wl_static", "fl_box",
-                            "fl_sum")) + [
+                            "fl_sum", "fl_chunk")) + [
         ("cap_f", c_int), ("cap_b", c_int), ("cap_d", c_int)]
 
 
@@ -99,7 +99,7 @@ class EisphNbr(ctypes.Structure):
 
 class TuningStruct(ctypes.Structure):
     _fields_ = [("block", c_int * NUM_PASSES), ("xcd_remap", c_int * NUM_PASSES),
-                ("por_x_tile", c_int), ("dem_beside", c_int)]
+                ("por_x_tile", c_int), ("dem_beside", c_int), ("por_parts", c_int)]
 
 
 _SIGNATURES = {
@@ -416,7 +416,8 @@ def get_tuning() -> dict:
     call("sphdem_tuning_get", t)
     return {"block": {n: int(t.block[i]) for i, n in enumerate(PASS_NAMES)},
             "xcd_remap": {n: int(t.xcd_remap[i]) for i, n in enumerate(PASS_NAMES)},
-            "por_x_tile": int(t.por_x_tile), "dem_beside": int(t.dem_beside)}
+            "por_x_tile": int(t.por_x_tile), "dem_beside": int(t.dem_beside),
+            "por_parts": int(t.por_parts)}
 
 
 def set_tuning(cfg: dict) -> None:
@@ -428,6 +429,7 @@ def set_tuning(cfg: dict) -> None:
         t.xcd_remap[i] = int(cfg.get("xcd_remap", {}).get(n, cur["xcd_remap"][n]))
     t.por_x_tile = int(cfg.get("por_x_tile", cur["por_x_tile"]))
     t.dem_beside = int(cfg.get("dem_beside", cur["dem_beside"]))
+    t.por_parts = int(cfg.get("por_parts", cur["por_parts"]))
     call("sphdem_tuning_set", t)
 
 

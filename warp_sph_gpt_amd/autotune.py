@@ -107,7 +107,7 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
 
 def reset() -> None:
     """Back to the library defaults (256 threads, no remap, x tile 4, the
-    DEM interpolation beside the porosity)."""
+    DEM interpolation beside the porosity, 4 porosity waves per work item)."""
     _lib.set_tuning({"block": {p: 256 for p in _lib.PASS_NAMES},
                      "xcd_remap": {p: 0 for p in _lib.PASS_NAMES}, "por_x_tile": 4,
-                     "dem_beside": 1})
+                     "dem_beside": 1, "por_parts": 4})

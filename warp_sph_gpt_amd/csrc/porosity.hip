@@ -264,14 +264,24 @@ __device__ __forceinline__ void mfma_blocks(f32x4 (&acc)[4], const float (&a)[4]
 }
 
 // ---- symmetric FP32 fast path
-// The wave's slot range, the flags, the fixed-point accumulators and their
-// scale (units per unit of the unscaled sum; a power of two).
+// The flags, each flagged slot's work item (tag: the chunk list's, or NULL:
+// slot / 64), the fixed-point accumulators, their scale (units per unit of
+// the unscaled sum; a power of two) and the wave's own item.  Pair rule of a
+// flagged candidate j: item(j) < cid: that item's pair (skipped here);
+// item(j) > cid: evaluated here for both sums; item(j) == cid: each subject
+// sums its own terms.
 struct SymCtx {
     const int *flag;
+    const int *tag;
     unsigned long long *col;
     float scale;
-    int w0, w1;
+    int cid;
 };
+
+// -1 for an unflagged slot
+__device__ __forceinline__ int item_of(const SymCtx &sym, int t) {
+    return sym.tag ? sym.tag[t] : (sym.flag[t] != 0 ? t >> 6 : -1);
+}
 
 // v >= 0 in fixed point: x = v scale (exact), split at 2^32 exactly (the
 // high part truncated, the low part an exact float difference by Sterbenz).
@@ -369,7 +379,7 @@ __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&
     auto load = [&](int s, float &v, int &f) {
         const int t = min(s, b0 - 1);
         v = Vs[t];
-        f = SYM ? sym.flag[t] : 0;
+        f = SYM ? item_of(sym, t) : -1;
         return pos[t];
     };
     float vcur;
@@ -388,10 +398,10 @@ __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&
         const float ez = fmaxf(fabsf(q.z) - he.z, 0.0f);
         const int s = s0 + lane;
         const bool near = s < b0 && fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= r2;
-        // SYM: a flagged candidate of a lower wave is that wave's pair; one
-        // of a higher wave goes to the symmetric queue
-        const bool higher = SYM && fcur != 0 && s >= sym.w1;
-        const bool keep = near && !(SYM && fcur != 0 && (s < sym.w0 || s >= sym.w1));
+        // SYM: a flagged candidate of a lower item is that item's pair; one
+        // of a higher item goes to the symmetric queue
+        const bool higher = SYM && fcur > sym.cid;
+        const bool keep = near && !(SYM && fcur >= 0 && fcur != sym.cid);
         q.w = fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x));
         const unsigned long long mask = __ballot(keep);
         if (keep) {
@@ -531,11 +541,12 @@ __device__ __forceinline__ T porosity_of(const Prm<T> &p, const DemV<T> &D, cons
 // Stream the union of the balls of the wave's lanes with `in` set; other
 // lanes compare against a zero threshold and add nothing.  Returns false
 // (having done nothing) if that union spans more than dims - 2 buckets of
-// the folded grid along some axis.
+// the folded grid along some axis.  FP32 FAST: only the bucket rows
+// part, part + parts, ... of the union's query (parts > 1: a partial sum).
 template <typename T, bool FAST, bool SYM>
 __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &gs, const T *Vs,
                              V4<T> *wbuf, T *vbuf, int lane, T csup, T thr2, T h, T inv_h,
-                             SymQueue &sq, T vme, const SymCtx &sym) {
+                             SymQueue &sq, T vme, const SymCtx &sym, int part, int parts) {
     const T big = std::numeric_limits<T>::max();
     const T bxl = uniform_f(wave_min(in ? me.x : big)), bxh = uniform_f(wave_max(in ? me.x : -big));
     const T byl = uniform_f(wave_min(in ? me.y : big)), byh = uniform_f(wave_max(in ? me.y : -big));
@@ -596,10 +607,11 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
         const V3<float> hef = mk3((float)((bxh - bxl) * (T)0.5), (float)((byh - byl) * (T)0.5),
                                   (float)((bzh - bzl) * (T)0.5));
         const int ny = Y1 - Y0 + 1, nrows = (Z1 - Z0 + 1) * ny;
-        for (int r0 = 0; r0 < nrows; r0 += 64) {
+        const int mine = part < nrows ? (nrows - part + parts - 1) / parts : 0;
+        for (int r0 = 0; r0 < mine; r0 += 64) {
             int ra1 = 0, rb1 = 0, ra2 = 0, rb2 = 0;
-            const int idx = r0 + lane;
-            if (idx < nrows) {
+            const int idx = (r0 + lane) * parts + part;
+            if (r0 + lane < mine) {
                 const int z = Z0 + idx / ny, y = Y0 + idx % ny;
                 const T gz = axis_gap(bzl, bzh, z, w);
                 const T gy = axis_gap(byl, byh, y, w);
@@ -613,7 +625,7 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
                               rb2);
                 }
             }
-            const int nr = min(64, nrows - r0);
+            const int nr = min(64, mine - r0);
             for (int r = 0; r < nr; ++r) {
                 const int a1 = __builtin_amdgcn_readlane(ra1, r);
                 const int b1 = __builtin_amdgcn_readlane(rb1, r);
@@ -690,41 +702,28 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
     return true;
 }
 
-// 2. one wave per 64 consecutive fluid slots.  The flagged lanes are
-// streamed in groups that share an instance and a bucket row (y, z) -- a
-// wave of sorted slots can straddle two rows -- and, when a row group is
-// still too wide (folded buckets), in sub-groups of 2^xt buckets along x; a
-// sub-group that still does not fit (grids narrower than the query) walks
-// its own query per lane.
+// 2. the stream.  A wave takes one work item: 64 near-DEM subjects of one
+// bucket row (the chunk list), or without the list 64 consecutive fluid
+// slots.  The wave's subjects are streamed in groups that share an instance
+// and a bucket row (y, z) -- 64 consecutive slots can straddle two rows --
+// and, when a row group is still too wide (folded buckets), in sub-groups of
+// 2^xt buckets along x; a sub-group that still does not fit (grids narrower
+// than the query) walks its own query per lane.
+// Lane subject: slot k where active; cid: the wave's work item (SYM's pair
+// rule), tag: the chunk of each flagged slot or NULL (slot / 64).  SYM: the
+// item's waves part = 0..parts-1 each stream their share of the bucket rows
+// and add their partial sums to the fixed-point accumulators (the per-lane
+// query of a group that fits no stream is part 0's).
 template <typename T, bool FAST, bool SYM>
-__global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
-                                                    GridV<T> gd, const T *Vs, const int *flag,
-                                                    T csup, T hu, T thr2, int xt, int remap,
-                                                    T *row_sum, unsigned long long *col_sum,
-                                                    float col_scale) {
-    // per wave: 64 staged candidates, 128 for the FP32 matrix-core queue;
-    // SYM: a second 128-entry queue for the pairs this wave evaluates for both
-    constexpr int kSlot = (FAST && std::is_same<T, float>::value) ? 128 : 64;
-    constexpr int kSymSlot = SYM ? 128 : 1;
-    __shared__ V4<T> stage[4 * kSlot];
-    __shared__ T vstage[4 * kSlot];
-    __shared__ V4<float> sstage[4 * kSymSlot];
-    __shared__ float svstage[4 * kSymSlot];
-    __shared__ int sidstage[4 * kSymSlot];
-    __shared__ __align__(16) float vsubstage[4 * (SYM ? 64 : 1)];
-    const int k = subject_id(remap);
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    V4<T> *wbuf = stage + wv * kSlot;
-    T *vbuf = vstage + wv * kSlot;
-    SymQueue sq{sstage + wv * kSymSlot, svstage + wv * kSymSlot, sidstage + wv * kSymSlot,
-                vsubstage + wv * (SYM ? 64 : 1), 0};
-    const int w0 = k & ~63;
-    const SymCtx sym{flag, col_sum, col_scale, w0, w0 + 64};
-    const bool active = k < gs.n && flag[k] != 0;
+__device__ void por_wave(const Prm<T> &p, const SphV<T> &P, const DemV<T> &D, const GridV<T> &gs,
+                         GridV<T> gd, const T *Vs, const int *flag, const int *tag, T csup, T hu,
+                         T thr2, int xt, unsigned long long *col_sum, float col_scale, int k,
+                         bool active, int cid, int part, int parts, int lane, V4<T> *wbuf,
+                         T *vbuf, SymQueue sq) {
+    const SymCtx sym{flag, tag, col_sum, col_scale, cid};
     const unsigned long long live = __ballot(active);
     if (live == 0ull) return;
-    const int lead = w0 + __ffsll((long long)live) - 1;
+    const int lead = __shfl(k, __ffsll((long long)live) - 1, 64);
     const int ks = active ? k : lead;
     const V4<T> me = gs.s[ks];
     const T vme = SYM ? Vs[ks] : (T)0;
@@ -742,7 +741,7 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
         const bool in = (grp >> lane) & 1ull;
         const GridV<T> gg = at_inst(gs, uniform_i(li));
         if (!stream_group<T, FAST, SYM>(flt, me, in, gg, Vs, wbuf, vbuf, lane, csup, thr2, h,
-                                        inv_h, sq, vme, sym)) {
+                                        inv_h, sq, vme, sym, part, parts)) {
             unsigned long long rest = grp;
             while (rest) {
                 const int l2 = __ffsll((long long)rest) - 1;
@@ -750,15 +749,16 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
                 const unsigned long long sub = __ballot(kx == lx) & rest;
                 const bool in2 = (sub >> lane) & 1ull;
                 if (!stream_group<T, FAST, SYM>(flt, me, in2, gg, Vs, wbuf, vbuf, lane, csup,
-                                                thr2, h, inv_h, sq, vme, sym) && in2) {
+                                                thr2, h, inv_h, sq, vme, sym, part, parts) &&
+                    in2 && part == 0) {
                     query_runs(at_inst(gs, ik), me.x, me.y, me.z, csup, [&](int lo, int hi) {
                         for (int s = lo; s < hi; ++s) {
                             const V4<T> q = gs.s[s];
                             if constexpr (SYM) {
                                 // the pair rule of the streamed groups, per pair
-                                const bool f = flag[s] != 0;
-                                if (f && s < sym.w0) continue;
-                                if (f && s >= sym.w1) {
+                                const int it = item_of(sym, s);
+                                if (it >= 0 && it < sym.cid) continue;
+                                if (it > sym.cid) {
                                     T w = (T)0;
                                     add_term<T, FAST>(w, me, q.x, q.y, q.z, (T)1, thr2, h,
                                                       inv_h);
@@ -780,26 +780,146 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
     }
     if (!active) return;
     if constexpr (SYM) {
-        // the subject's own share; k_por_finish adds the mirrored terms
-        row_sum[k] = flt;
+        // the subject's own share; k_por_finish reads it with the mirrored terms
+        if (flt > (T)0) atomicAdd(col_sum + k, to_fixed((float)flt, col_scale));
         return;
     }
     if (FAST) flt = flt * ((T)21.0 / ((T)16.0 * (T)SPHDEM_PI * hu * hu * hu));
     P.porosity[gs.ids[k]] = porosity_of(p, D, gd, me, flt, csup);
 }
 
-// SYM: each flagged subject's sum is its own share plus the mirrored terms
-// the lower waves added (read, then cleared for the next step), then the
+// The chunk list (CHUNKS): cl = {list, L, tag}, count = {chunks, flagged};
+// wave c of the grid takes chunks c, c + waves, ...
+struct ChunkV {
+    const int *list, *L, *tag, *count;
+};
+
+template <typename T, bool FAST, bool SYM, bool CHUNKS>
+__global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
+                                                    GridV<T> gd, const T *Vs, const int *flag,
+                                                    T csup, T hu, T thr2, int xt, int remap,
+                                                    unsigned long long *col_sum,
+                                                    float col_scale, ChunkV cl, int parts) {
+    // per wave: 64 staged candidates, 128 for the FP32 matrix-core queue;
+    // SYM: a second 128-entry queue for the pairs this wave evaluates for both
+    constexpr int kSlot = (FAST && std::is_same<T, float>::value) ? 128 : 64;
+    constexpr int kSymSlot = SYM ? 128 : 1;
+    __shared__ V4<T> stage[4 * kSlot];
+    __shared__ T vstage[4 * kSlot];
+    __shared__ V4<float> sstage[4 * kSymSlot];
+    __shared__ float svstage[4 * kSymSlot];
+    __shared__ int sidstage[4 * kSymSlot];
+    __shared__ __align__(16) float vsubstage[4 * (SYM ? 64 : 1)];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    V4<T> *wbuf = stage + wv * kSlot;
+    T *vbuf = vstage + wv * kSlot;
+    const SymQueue sq{sstage + wv * kSymSlot, svstage + wv * kSymSlot, sidstage + wv * kSymSlot,
+                      vsubstage + wv * (SYM ? 64 : 1), 0};
+    const int wave = subject_id(remap) >> 6;
+    if constexpr (CHUNKS) {
+        // items (chunk, part), the parts of a chunk on neighbouring waves
+        const int waves = (gridDim.x * blockDim.x) >> 6;
+        const int nch = cl.count[0], nfl = cl.count[1];
+        for (int item = wave; item < nch * parts; item += waves) {
+            const int c = item / parts, part = item - c * parts;
+            const int i = cl.list[c] + lane;
+            const bool active = i < (c + 1 < nch ? cl.list[c + 1] : nfl);
+            const int k = active ? cl.L[i] : 0;
+            por_wave<T, FAST, SYM>(p, P, D, gs, gd, Vs, flag, cl.tag, csup, hu, thr2, xt,
+                                   col_sum, col_scale, k, active, c, part, parts, lane, wbuf,
+                                   vbuf, sq);
+        }
+    } else {
+        const int k = wave * 64 + lane;
+        const bool active = k < gs.n && flag[k] != 0;
+        por_wave<T, FAST, SYM>(p, P, D, gs, gd, Vs, flag, nullptr, csup, hu, thr2, xt, col_sum,
+                               col_scale, k, active, wave, 0, 1, lane, wbuf, vbuf, sq);
+    }
+}
+
+// ---- the chunk list: the near-DEM slots of each bucket row (instance,
+// unfolded y and z cell: the groups the stream forms), compacted in slot
+// order and cut every 64.  Arrays of n ints (fl_chunk, 7 of them):
+//   F   exclusive scan of the flags; L[F[k]] = k: the flagged slots
+//   R   exclusive scan of the row heads of L (R[i] + head(i) - 1: row of i)
+//   RF  first index of L of each row
+//   CS  exclusive scan of the chunk starts (i - RF[row]) % 64 == 0
+//   list[chunk] = its first index of L;  tag[k] = chunk of flagged slot k
+//   (-1 unflagged);  count[0] = chunks, count[1] = flagged slots.
+template <typename T>
+__device__ __forceinline__ bool row_head(const GridV<T> &gs, const int *L, int i) {
+    if (i == 0) return true;
+    const int k = L[i], j = L[i - 1];
+    const V4<T> a = gs.s[k], b = gs.s[j];
+    const T inv = gs.inv;
+    return inst_of(gs, k) != inst_of(gs, j) || (int)(a.y * inv) != (int)(b.y * inv) ||
+           (int)(a.z * inv) != (int)(b.z * inv);
+}
+
+__global__ void k_cl_compact(int n, const int *flag, const int *F, int *L, int *tag,
+                             int *count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (flag[k] != 0) L[F[k]] = k;
+    else tag[k] = -1;
+    if (k == n - 1) count[1] = F[k] + (flag[k] != 0 ? 1 : 0);
+}
+
+template <typename T>
+__global__ void k_cl_heads(GridV<T> gs, const int *L, const int *count, int *H) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < gs.n) H[i] = (i < count[1] && row_head(gs, L, i)) ? 1 : 0;
+}
+
+template <typename T>
+__global__ void k_cl_firsts(GridV<T> gs, const int *L, const int *count, const int *R, int *RF) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count[1] && row_head(gs, L, i)) RF[R[i]] = i;
+}
+
+template <typename T>
+__device__ __forceinline__ bool chunk_start(const GridV<T> &gs, const int *L, const int *R,
+                                            const int *RF, int i) {
+    const int row = R[i] + (row_head(gs, L, i) ? 1 : 0) - 1;
+    return ((i - RF[row]) & 63) == 0;
+}
+
+template <typename T>
+__global__ void k_cl_marks(GridV<T> gs, const int *L, const int *count, const int *R,
+                           const int *RF, int *C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < gs.n) C[i] = (i < count[1] && chunk_start(gs, L, R, RF, i)) ? 1 : 0;
+}
+
+template <typename T>
+__global__ void k_cl_list(GridV<T> gs, const int *L, const int *R, const int *RF, const int *CS,
+                          int *list, int *tag, int *count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nfl = count[1];
+    if (i == 0 && nfl == 0) count[0] = 0;
+    if (i >= nfl) return;
+    const bool st = chunk_start(gs, L, R, RF, i);
+    const int c = CS[i] + (st ? 1 : 0) - 1;
+    if (st) list[c] = i;
+    tag[L[i]] = c;
+    if (i == nfl - 1) count[0] = c + 1;
+}
+
+// SYM: each flagged subject's sum is the fixed-point total of its own
+// partial sums and the mirrored terms the lower items added (read, then
+// cleared for the next step; the unscaled sum is left in row_sum), then the
 // porosity as k_por_stream finishes it.
 template <typename T>
 __global__ void k_por_finish(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs, GridV<T> gd,
-                             const int *flag, const T *row_sum, unsigned long long *col_sum,
+                             const int *flag, T *row_sum, unsigned long long *col_sum,
                              double inv_scale, T csup, T hu, int remap) {
     const int k = subject_id(remap);
     if (k >= gs.n || flag[k] == 0) return;
     const unsigned long long cv = col_sum[k];
     col_sum[k] = 0ull;
-    T flt = (T)((double)row_sum[k] + (double)cv * inv_scale);
+    T flt = (T)((double)cv * inv_scale);
+    row_sum[k] = flt;
     flt = flt * ((T)21.0 / ((T)16.0 * (T)SPHDEM_PI * hu * hu * hu));
     const V4<T> me = gs.s[k];
     gd = at_inst(gd, inst_of(gs, k));
@@ -833,29 +953,63 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
     // the flags are final here: work that needs only them may start
     if (marked) hipEventRecord(marked, st);
     const T thr2 = sqrt_threshold<T>(cs);
+    // the chunk list of the flagged slots (fl_chunk: F, L, R, RF, CS, list,
+    // tag of n ints each; the counts in fl_count[0..1])
+    ChunkV cl{nullptr, nullptr, nullptr, nullptr};
+    const bool chunked = w->fl_chunk != nullptr;
+    if (chunked) {
+        int *F = w->fl_chunk, *L = F + n, *R = L + n, *RF = R + n, *CS = RF + n, *list = CS + n,
+            *tag = list + n;
+        const dim3 g(blocks_for(n, 256)), b(256);
+        int rc = exclusive_scan(w->fl_flag, F, n, w->fl_scan, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_cl_compact, g, b, 0, st, n, w->fl_flag, F, L, tag, w->fl_count);
+        hipLaunchKernelGGL(k_cl_heads<T>, g, b, 0, st, view<T>(gs), L, w->fl_count, R);
+        if ((rc = exclusive_scan(R, R, n, w->fl_scan, st))) return rc;
+        hipLaunchKernelGGL(k_cl_firsts<T>, g, b, 0, st, view<T>(gs), L, w->fl_count, R, RF);
+        hipLaunchKernelGGL(k_cl_marks<T>, g, b, 0, st, view<T>(gs), L, w->fl_count, R, RF, CS);
+        if ((rc = exclusive_scan(CS, CS, n, w->fl_scan, st))) return rc;
+        hipLaunchKernelGGL(k_cl_list<T>, g, b, 0, st, view<T>(gs), L, R, RF, CS, list, tag,
+                           w->fl_count);
+        cl = ChunkV{list, L, tag, w->fl_count};
+    }
     const bool sym = p->fast_math && std::is_same<T, float>::value && w->fl_sum;
     if (sym) {
         // fixed-point units: the unscaled sum of a full ball (1/C) near 2^40
         const int e = 40 - (int)std::ceil(std::log2(16.0 * SPHDEM_PI * hu * hu * hu / 21.0));
         const float scale = std::ldexp(1.0f, e);
         auto *col = reinterpret_cast<unsigned long long *>(w->fl_list);
-        hipLaunchKernelGGL((k_por_stream<T, true, std::is_same<T, float>::value>),
-                           dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p), view<T>(P),
-                           view<T>(D), view<T>(gs), view<T>(gd), (const T *)w->fl_V, w->fl_flag,
-                           cs, (T)hu, thr2, xt, rm, (T *)w->fl_sum, col, scale);
+        constexpr bool kF = std::is_same<T, float>::value;
+        // waves per chunk (the grid's waves loop over the items if fewer)
+        const int parts = tuning().por_parts;
+        if (chunked)
+            hipLaunchKernelGGL((k_por_stream<T, true, kF, true>), dim3(blocks_for(n, bt)),
+                               dim3(bt), 0, st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs),
+                               view<T>(gd), (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt,
+                               rm, col, scale, cl, parts);
+        else
+            hipLaunchKernelGGL((k_por_stream<T, true, kF, false>), dim3(blocks_for(n, bt)),
+                               dim3(bt), 0, st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs),
+                               view<T>(gd), (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt,
+                               rm, col, scale, cl, 1);
         hipLaunchKernelGGL(k_por_finish<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
                            view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), w->fl_flag,
-                           (const T *)w->fl_sum, col, std::ldexp(1.0, -e), cs, (T)hu, rm);
-    } else if (p->fast_math) {
-        hipLaunchKernelGGL((k_por_stream<T, true, false>), dim3(blocks_for(n, bt)), dim3(bt), 0,
-                           st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm, nullptr,
-                           nullptr, 0.0f);
+                           (T *)w->fl_sum, col, std::ldexp(1.0, -e), cs, (T)hu, rm);
     } else {
-        hipLaunchKernelGGL((k_por_stream<T, false, false>), dim3(blocks_for(n, bt)), dim3(bt), 0,
-                           st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                           (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm, nullptr,
-                           nullptr, 0.0f);
+        // FAST without the symmetric sums, or EXACT; with or without chunks
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
+                               view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
+                               (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm,
+                               (unsigned long long *)nullptr, 0.0f, cl, 1);
+        };
+        if (p->fast_math) {
+            if (chunked) launch(k_por_stream<T, true, false, true>);
+            else launch(k_por_stream<T, true, false, false>);
+        } else {
+            if (chunked) launch(k_por_stream<T, false, false, true>);
+            else launch(k_por_stream<T, false, false, false>);
+        }
     }
     return check_launch("porosity");
 }

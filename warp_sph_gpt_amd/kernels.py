@@ -48,6 +48,7 @@ class Workspace:
             # fixed-point accumulators of the mirrored terms, zero between steps
             self.fl_sum = torch.zeros(c, dtype=t, device=d) if t == torch.float32 else None
             self.fl_list = torch.zeros(2 * c, dtype=torch.int32, device=d)
+            self.fl_chunk = torch.zeros(7 * c, dtype=torch.int32, device=d)
             self.fl_count = torch.zeros(4, dtype=torch.int32, device=d)
             self.fl_scan = torch.zeros(_lib.load().sphdem_scan_scratch(c), dtype=torch.int32,
                                        device=d)
@@ -86,6 +87,7 @@ class Workspace:
         w.wl_static = self.wl_static.data_ptr() if self.wl_static is not None else None
         w.fl_box = self.fl_box.data_ptr() if self.fl_box is not None else None
         w.fl_sum = self.fl_sum.data_ptr() if self.fl_sum is not None else None
+        w.fl_chunk = self.fl_chunk.data_ptr() if self.fl_chunk is not None else None
         w.cap_f, w.cap_b, w.cap_d = self.cap_f, self.cap_b, self.cap_d
         return w
 
