@@ -782,18 +782,20 @@ static int surrogate_prep(const sphdem_params_t *p, const sphdem_sph_t *P, const
     return check_launch("surrogate_prep");
 }
 
-// n_sub DEM substeps of a surrogate call in ONE workgroup (all DEM, the
-// substeps separated by workgroup barriers): per substep the DEM-DEM
-// contacts (candidates from the DEM grid built at the call's start, queried
-// with the radius widened by `skin` and read at their current positions),
-// the DEM-wall contacts (static walls), the coupling force held from the
-// call's start, and the integration of Kernel_step_dem.  Contact histories
-// live in fixed-width rows (kEll per DEM and kind) in `scratch` and go back
-// to the CSR (old and new, as stage -1 leaves them) at the end.  Loads are
-// the same as the plain DEM stage's (dem_contact, gravity, last-match
-// history); flags bit 0: CSR capacity exceeded, bit 1: a DEM moved farther
-// than `skin` from where the grid was built, bit 2: a history row exceeded
-// kEll contacts.
+// n_sub DEM substeps of a surrogate call: per substep the DEM-DEM contacts
+// (candidates from the DEM grid built at the call's start, queried with the
+// radius widened by `skin` and read at their current positions), the
+// DEM-wall contacts (static walls), the coupling force held from the call's
+// start, then the integration of Kernel_step_dem.  Contact histories live in
+// fixed-width rows (kEll per DEM and kind) in `scratch` and go back to the
+// CSR (old and new, as stage -1 leaves them) at the end.  Loads are the same
+// as the plain DEM stage's (dem_contact, gravity, last-match history) and are
+// summed in the same candidate order; flags bit 0: CSR capacity exceeded,
+// bit 1: a DEM moved farther than `skin` from where the grid was built, bit
+// 2: a history row exceeded kEll contacts.
+// Launches: init; per substep loads (one wave per DEM: the lanes test 64
+// candidates at a time, the contacts are compacted in candidate order and
+// lane 0 adds them in that order) and integration; the CSR write-back.
 constexpr int kEll = 64;
 constexpr int kSubThreads = 1024;
 
@@ -822,65 +824,82 @@ __device__ __forceinline__ void block_scan_rows(const int *cnt, int n, int *off,
     __syncthreads();
 }
 
+// Substep scratch: cnt/ids/tang rows [side 0/1][kind 0 = dem, 1 = wall][n][kEll].
+template <typename T> struct SubRows {
+    T *pos0;
+    int *cnt, *ids;
+    T *tang;
+    int n;
+    __device__ int row(int side, int kind, int a) const { return (side * 2 + kind) * n + a; }
+};
+
+// CSR (old) -> rows of side 0; the positions the grid was built from.
 template <typename T>
-__global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> D, WallV<T> W,
-                                                              GridV<T> gd, GridV<T> gw,
-                                                              const int *dinst, int n_sub, T skin,
-                                                              T *pos0, int *cnt, int *ids,
-                                                              T *tang, int *flags) {
-    const int n = D.n;
-    // cnt/ids/tang: [side 0/1][kind 0 = dem, 1 = wall][n][kEll]
-    auto row = [&](int side, int kind, int a) { return ((side * 2 + kind) * n + a); };
-    // 0. CSR (old) -> rows of side 0; positions the grid was built from
-    for (int a = threadIdx.x; a < n; a += kSubThreads) {
-        st3(pos0, a, ld3(D.pos, a));
-        for (int kind = 0; kind < 2; ++kind) {
-            const int *off = kind ? D.bnd_off_old : D.dem_off_old;
-            const int *id = kind ? D.bnd_id_old : D.dem_id_old;
-            const T *tg = kind ? D.bnd_tang_old : D.dem_tang_old;
-            const int o0 = off[a], o1 = off[a + 1];
-            int c = 0;
-            for (int k = o0; k < o1; ++k) {
-                if (c == kEll) {
-                    atomicOr(flags, 4);
-                    break;
-                }
-                const int r = row(0, kind, a) * kEll + c;
-                ids[r] = id[k];
-                st3(tang, r, ld3(tg, k));
-                ++c;
+__global__ void k_sub_init(DemV<T> D, SubRows<T> R, int *flags) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= R.n) return;
+    st3(R.pos0, a, ld3(D.pos, a));
+    for (int kind = 0; kind < 2; ++kind) {
+        const int *off = kind ? D.bnd_off_old : D.dem_off_old;
+        const int *id = kind ? D.bnd_id_old : D.dem_id_old;
+        const T *tg = kind ? D.bnd_tang_old : D.dem_tang_old;
+        const int o0 = off[a], o1 = off[a + 1];
+        int c = 0;
+        for (int k = o0; k < o1; ++k) {
+            if (c == kEll) {
+                atomicOr(flags, 4);
+                break;
             }
-            cnt[row(0, kind, a)] = c;
+            const int r = R.row(0, kind, a) * kEll + c;
+            R.ids[r] = id[k];
+            st3(R.tang, r, ld3(tg, k));
+            ++c;
         }
+        R.cnt[R.row(0, kind, a)] = c;
     }
-    __syncthreads();
-    for (int sub = 0; sub < n_sub; ++sub) {
-        const int cur = sub & 1, nxt = cur ^ 1;
-        // A. loads (reads neighbours' current state; writes force/torque and rows of nxt)
-        for (int a = threadIdx.x; a < n; a += kSubThreads) {
-            const int ia = dinst ? dinst[a] : 0;
-            const V3<T> ra = ld3(D.pos, a);
-            const T Ra = D.radius[a];
-            const V3<T> va = ld3(D.vel, a), wa = ld3(D.omega, a);
-            const T K = D.K[a], eta = D.eta[a], mu = D.mu[a];
-            // DEM-DEM loads start from gravity; the wall loads are summed
-            // apart and added, as the two plain kernels do
-            V3<T> fk[2] = {mk3<T>(0, 0, -D.m[a] * p.g), mk3<T>(0, 0, 0)};
-            V3<T> tk[2] = {mk3<T>(0, 0, 0), mk3<T>(0, 0, 0)};
-            for (int kind = 0; kind < 2; ++kind) {
-                V3<T> &force = fk[kind], &torque = tk[kind];
-                const GridV<T> g = at_inst(kind ? gw : gd, ia);
-                const T *nr = kind ? W.radius : D.radius;
-                const T *nv = kind ? W.vel : D.vel;
-                const T *nw = kind ? W.omega : D.omega;
-                const int rc = row(cur, kind, a), rn = row(nxt, kind, a);
-                const int c0 = cnt[rc];
-                int out = 0;
-                const T reach = Ra + (kind ? p.boundary_radius_max : p.radius_max + skin);
-                query_runs(g, ra.x, ra.y, ra.z, reach, [&](int lo, int hi) {
-                    for (int s = lo; s < hi; ++s) {
-                        const int b = g.ids[s];
-                        if (!kind && b == a) continue;
+}
+
+// Loads of one substep: reads the neighbours' current state and the rows of
+// side `cur`, writes D.force/torque and the rows of the other side.
+constexpr int kSubWaves = 4;
+template <typename T>
+__global__ __launch_bounds__(64 * kSubWaves) void k_sub_loads(Prm<T> p, DemV<T> D, WallV<T> W,
+                                                              GridV<T> gd, GridV<T> gw,
+                                                              const int *dinst, T skin, int cur,
+                                                              SubRows<T> R, int *flags) {
+    __shared__ V3<T> fbuf[kSubWaves][64], tbuf[kSubWaves][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int a = blockIdx.x * kSubWaves + wv;
+    if (a >= R.n) return;
+    const int nxt = cur ^ 1;
+    const int ia = dinst ? dinst[a] : 0;
+    const V3<T> ra = ld3(D.pos, a);
+    const T Ra = D.radius[a];
+    const V3<T> va = ld3(D.vel, a), wa = ld3(D.omega, a);
+    const T K = D.K[a], eta = D.eta[a], mu = D.mu[a];
+    // DEM-DEM loads start from gravity; the wall loads are summed apart and
+    // added, as the two plain kernels do
+    V3<T> fk[2] = {mk3<T>(0, 0, -D.m[a] * p.g), mk3<T>(0, 0, 0)};
+    V3<T> tk[2] = {mk3<T>(0, 0, 0), mk3<T>(0, 0, 0)};
+    for (int kind = 0; kind < 2; ++kind) {
+        const GridV<T> g = at_inst(kind ? gw : gd, ia);
+        const T *nr = kind ? W.radius : D.radius;
+        const T *nv = kind ? W.vel : D.vel;
+        const T *nw = kind ? W.omega : D.omega;
+        const int rc = R.row(cur, kind, a), rn = R.row(nxt, kind, a);
+        const int c0 = R.cnt[rc];
+        int out = 0;
+        V3<T> force = fk[kind], torque = tk[kind];
+        const T reach = Ra + (kind ? p.boundary_radius_max : p.radius_max + skin);
+        query_runs(g, ra.x, ra.y, ra.z, reach, [&](int lo, int hi) {
+            for (int base = lo; base < hi; base += 64) {
+                const int s = base + lane;
+                bool hit = false;
+                int b = 0;
+                V3<T> f = mk3<T>(0, 0, 0), disp = f, tq = f;
+                if (s < hi) {
+                    b = g.ids[s];
+                    if (kind || b != a) {
                         V3<T> rb;
                         if (kind) {
                             const V4<T> q = g.s[s];
@@ -891,49 +910,76 @@ __global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> 
                         const V3<T> rab = rb - ra;
                         const T dist = length(rab);
                         const T overlap = Ra + nr[b] - dist;
-                        if (!(overlap > (T)0)) continue;
-                        V3<T> tangent = mk3<T>(0, 0, 0);
-                        for (int k = 0; k < c0; ++k)
-                            if (ids[rc * kEll + k] == b) tangent = ld3(tang, rc * kEll + k);
-                        V3<T> normal = kind ? mk3<T>(0, 0, -1) : mk3<T>(1, 0, 0);
-                        if (!kind && a > b) normal = -normal;
-                        if (dist > (T)1.0e-12) normal = rab / dist;
-                        V3<T> f, disp, tq;
-                        dem_contact(normal, overlap, Ra, nr[b], va, ld3(nv, b), wa, ld3(nw, b),
-                                    tangent, K, eta, mu, p.dt, f, disp, tq);
-                        force = force + f;
-                        torque = torque + tq;
-                        if (out < kEll) {
-                            ids[rn * kEll + out] = b;
-                            st3(tang, rn * kEll + out, disp);
-                        } else {
-                            atomicOr(flags, 4);
+                        if (overlap > (T)0) {
+                            hit = true;
+                            V3<T> tangent = mk3<T>(0, 0, 0);
+                            for (int k = 0; k < c0; ++k)
+                                if (R.ids[rc * kEll + k] == b)
+                                    tangent = ld3(R.tang, rc * kEll + k);
+                            V3<T> normal = kind ? mk3<T>(0, 0, -1) : mk3<T>(1, 0, 0);
+                            if (!kind && a > b) normal = -normal;
+                            if (dist > (T)1.0e-12) normal = rab / dist;
+                            dem_contact(normal, overlap, Ra, nr[b], va, ld3(nv, b), wa,
+                                        ld3(nw, b), tangent, K, eta, mu, p.dt, f, disp, tq);
                         }
-                        ++out;
                     }
-                });
-                cnt[rn] = out < kEll ? out : kEll;
+                }
+                const unsigned long long m = __ballot(hit);
+                const int before = (int)__builtin_amdgcn_mbcnt_hi(
+                    (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                if (hit) {
+                    const int o = out + before;
+                    if (o < kEll) {
+                        R.ids[rn * kEll + o] = b;
+                        st3(R.tang, rn * kEll + o, disp);
+                    } else {
+                        atomicOr(flags, 4);
+                    }
+                    fbuf[wv][before] = f;
+                    tbuf[wv][before] = tq;
+                }
+                const int nh = __popcll(m);
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0)
+                    for (int j = 0; j < nh; ++j) {
+                        force = force + fbuf[wv][j];
+                        torque = torque + tbuf[wv][j];
+                    }
+                __builtin_amdgcn_wave_barrier();
+                out += nh;
             }
-            const V3<T> fc = ((fk[0] + fk[1]) + ld3(D.drag, a)) + ld3(D.pressure_force, a);
-            st3(D.force, a, fc);
-            st3(D.torque, a, tk[0] + tk[1]);
-        }
-        __syncthreads();
-        // B. integration (Kernel_step_dem)
-        for (int a = threadIdx.x; a < n; a += kSubThreads) {
-            const V3<T> acc = ld3(D.force, a) / D.m[a];
-            const V3<T> vel = ld3(D.vel, a) + acc * p.dt;
-            const V3<T> pos = ld3(D.pos, a) + vel * p.dt;
-            st3(D.acc, a, acc);
-            st3(D.vel, a, vel);
-            st3(D.pos, a, pos);
-            st3(D.omega, a, ld3(D.omega, a) + (ld3(D.torque, a) / D.inertia[a]) * p.dt);
-            if (length(pos - ld3(pos0, a)) > skin) atomicOr(flags, 2);
-        }
-        __syncthreads();
+        });
+        fk[kind] = force;
+        tk[kind] = torque;
+        if (lane == 0) R.cnt[rn] = out < kEll ? out : kEll;
     }
-    // rows of the last substep -> CSR, written to both old and new
-    const int fin = n_sub & 1;
+    if (lane == 0) {
+        const V3<T> fc = ((fk[0] + fk[1]) + ld3(D.drag, a)) + ld3(D.pressure_force, a);
+        st3(D.force, a, fc);
+        st3(D.torque, a, tk[0] + tk[1]);
+    }
+}
+
+// Integration of one substep (Kernel_step_dem) and the skin check.
+template <typename T>
+__global__ void k_sub_step(DemV<T> D, SubRows<T> R, T dt, T skin, int *flags) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= R.n) return;
+    const V3<T> acc = ld3(D.force, a) / D.m[a];
+    const V3<T> vel = ld3(D.vel, a) + acc * dt;
+    const V3<T> pos = ld3(D.pos, a) + vel * dt;
+    st3(D.acc, a, acc);
+    st3(D.vel, a, vel);
+    st3(D.pos, a, pos);
+    st3(D.omega, a, ld3(D.omega, a) + (ld3(D.torque, a) / D.inertia[a]) * dt);
+    if (length(pos - ld3(R.pos0, a)) > skin) atomicOr(flags, 2);
+}
+
+// Rows of the last substep (side fin) -> CSR, written to both old and new.
+template <typename T>
+__global__ __launch_bounds__(kSubThreads) void k_sub_final(DemV<T> D, SubRows<T> R, int fin,
+                                                           int *flags) {
+    const int n = R.n;
     for (int kind = 0; kind < 2; ++kind) {
         int *off_o = kind ? D.bnd_off_old : D.dem_off_old;
         int *off_n = kind ? D.bnd_off_new : D.dem_off_new;
@@ -943,20 +989,20 @@ __global__ __launch_bounds__(kSubThreads) void k_dem_substeps(Prm<T> p, DemV<T> 
         T *tg_n = kind ? D.bnd_tang_new : D.dem_tang_new;
         const int cap = kind ? D.cap_bnd_new : D.cap_dem_new;
         __shared__ int total;
-        block_scan_rows<T>(cnt + row(fin, kind, 0), n, off_n, &total);
+        block_scan_rows<T>(R.cnt + R.row(fin, kind, 0), n, off_n, &total);
         // the read side's offsets are clamped to the capacity (as k_copy_csr
         // clamps them), so an overflow leaves the next lookups in bounds
         for (int a = threadIdx.x; a < n; a += kSubThreads) {
             const int o = off_n[a];
             off_o[a] = min(o, cap);
-            const int r = row(fin, kind, a);
-            for (int k = 0; k < cnt[r]; ++k) {
+            const int r = R.row(fin, kind, a);
+            for (int k = 0; k < R.cnt[r]; ++k) {
                 if (o + k >= cap) {
                     atomicOr(flags, 1);
                     break;
                 }
-                const int id = ids[r * kEll + k];
-                const V3<T> tv = ld3(tang, r * kEll + k);
+                const int id = R.ids[r * kEll + k];
+                const V3<T> tv = ld3(R.tang, r * kEll + k);
                 id_o[o + k] = id;
                 id_n[o + k] = id;
                 st3(tg_o, o + k, tv);
@@ -977,17 +1023,27 @@ static int dem_substeps(const sphdem_params_t *p, const sphdem_dem_t *D, const s
                         void *scratch, const sphdem_work_t *w, hipStream_t st) {
     const int n = D->n;
     char *sp = (char *)scratch;
-    T *pos0 = (T *)sp;
+    SubRows<T> R;
+    R.n = n;
+    R.pos0 = (T *)sp;
     sp += sizeof(T) * 3 * (size_t)n;
-    int *cnt = (int *)sp;
+    R.cnt = (int *)sp;
     sp += sizeof(int) * 4 * (size_t)n;
-    int *ids = (int *)sp;
+    R.ids = (int *)sp;
     sp += sizeof(int) * 4 * (size_t)n * kEll;
-    T *tang = (T *)sp;
+    R.tang = (T *)sp;
     ProfScope prof(SPHDEM_PASS_DEM_CONTACT, st);
-    hipLaunchKernelGGL(k_dem_substeps<T>, dim3(1), dim3(kSubThreads), 0, st, prm<T>(p), view<T>(D),
-                       view<T>(W), view<T>(gd), view<T>(gw), dem_inst(gd), n_sub, (T)skin, pos0,
-                       cnt, ids, tang, w->flags);
+    const DemV<T> dv = view<T>(D);
+    const dim3 g1(blocks_for(n, 256)), b1(256);
+    hipLaunchKernelGGL(k_sub_init<T>, g1, b1, 0, st, dv, R, w->flags);
+    for (int sub = 0; sub < n_sub; ++sub) {
+        hipLaunchKernelGGL(k_sub_loads<T>, dim3(blocks_for(n, kSubWaves)), dim3(64 * kSubWaves),
+                           0, st, prm<T>(p), dv, view<T>(W), view<T>(gd), view<T>(gw),
+                           dem_inst(gd), (T)skin, sub & 1, R, w->flags);
+        hipLaunchKernelGGL(k_sub_step<T>, g1, b1, 0, st, dv, R, (T)p->dt, (T)skin, w->flags);
+    }
+    hipLaunchKernelGGL(k_sub_final<T>, dim3(1), dim3(kSubThreads), 0, st, dv, R, n_sub & 1,
+                       w->flags);
     return check_launch("dem_substeps");
 }
 
