@@ -72,7 +72,7 @@ def main(argv=None) -> int:
     ap.add_argument("--instances", type=int, default=64, help="c3: number of instances")
     ap.add_argument("--tune-steps", type=int, default=2,
                     help="steps per autotuner candidate during warm-up (0: library defaults)")
-    ap.add_argument("--surrogate-sim-time", type=float, default=0.1,
+    ap.add_argument("--surrogate-sim-time", type=float, default=0.5,
                     help="simulated seconds of the C1 surrogate-vs-plain rollout (0: skip)")
     ap.add_argument("--math", choices=("exact", "fast"), default="fast",
                     help="W/dW arithmetic of the neighbour sums (parity-tested both ways)")
@@ -201,8 +201,9 @@ def main(argv=None) -> int:
             # the pass is bound by vector-ALU issue, not by HBM bytes: its
             # FP32 rate against the vector FP32 peak, and the issue fraction
             tflops = pairs * POROSITY_FLOP_PER_PAIR / avg_s / 1e12
-            roofline["limiter"] = {"what": "valu_issue", "frac": c["valu_issue_frac"],
-                                   "source": src}
+            roofline["limiter"] = {"what": "valu_issue (model: 4 cycles per wave64 VALU "
+                                           "instruction, 16 per square root)",
+                                   "frac": c["valu_issue_frac"], "source": src}
             roofline["valu"] = {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
                                 "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
                                 "flop_per_pair": POROSITY_FLOP_PER_PAIR,
@@ -326,8 +327,8 @@ def surrogate_rollout(sim_time: float) -> dict:
     net = MPNN(params=load_params(WEIGHTS))
     r = rollout_c1(net, sim_time=sim_time, judge_every=50, density_tol=0.01)
     keep = ("sim_time", "calls", "speedup", "plain_sim_per_wall", "surrogate_sim_per_wall",
-            "surrogate_calls", "plain_calls", "judged", "rejected", "velocity_err_mean",
-            "velocity_err_max", "density_err_mean", "density_err_max")
+            "surrogate_calls", "plain_calls", "judged", "rejected", "unhealthy",
+            "velocity_err_mean", "velocity_err_max", "density_err_mean", "density_err_max")
     out = {k: (round(r[k], 6) if isinstance(r[k], float) and r[k] == r[k]
                and abs(r[k]) != float("inf") else (None if isinstance(r[k], float) else r[k]))
            for k in keep}
@@ -336,6 +337,19 @@ def surrogate_rollout(sim_time: float) -> dict:
         / r["final_plain"]["fluid_ke"]
     out["final_front_rel"] = abs(r["final_surrogate"]["front"] - r["final_plain"]["front"]) \
         / r["final_plain"]["front"]
+    de, ve = out["density_err_max"], out["velocity_err_max"]
+    out["bar"] = {"speedup": 5.0, "density_err": 0.01, "velocity_err": 0.02,
+                  "met": bool(out["speedup"] >= 5.0 and de is not None and de <= 0.01
+                              and ve is not None and ve <= 0.02)}
+    # what a call costs (stages of an eager call, a replayed call, ten plain
+    # substeps replayed) and the network kernels' matrix-core use from their
+    # counter profile
+    out["call_costs"] = r.get("call_costs")
+    mf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                      "surrogate_c1_mfma.json")
+    if os.path.exists(mf):
+        with open(mf) as f:
+            out["network_mfma"] = dict(json.load(f), source="profiles/surrogate_c1_mfma.json")
     log(f"[bench] surrogate: {json.dumps(out)}")
     return out
 

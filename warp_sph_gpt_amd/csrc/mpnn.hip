@@ -46,9 +46,9 @@ constexpr float kSumScale = 1.0f / 64.0f;
 constexpr float kUScale = 1.0f;     // velocity scale of the features, m/s
 constexpr float kAScale = 9.81f;    // acceleration scale, m/s^2
 
+// float -> bf16, round to nearest even (one v_cvt_pk_bf16_f32)
 __device__ __forceinline__ unsigned short f2bf(float x) {
-    const unsigned int b = __float_as_uint(x);
-    return (unsigned short)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+    return __builtin_bit_cast(unsigned short, (__bf16)x);
 }
 __device__ __forceinline__ float bf2f(unsigned short h) {
     return __uint_as_float(((unsigned int)h) << 16);

@@ -548,25 +548,89 @@ class SurrogateRollout:
         self._flags = torch.zeros(4, dtype=torch.int32, device=e.device)
 
     # ---- one call of each kind
-    def _enqueue_surrogate(self, step: int) -> None:
+    def _enqueue_surrogate(self, step: int, mark=None) -> None:
+        """Enqueue one call; `mark(stage)` (if given) is called after each
+        stage: grids, prep, dem, network, apply."""
         e = self.eng
         s = self.solv
+        mark = mark or (lambda stage: None)
         st = _lib.stream_handle()
         e.gs.build(e.P.pos, s.support, weights=e.P.m)
         dem = _lib.dem_view(e.D) if e.coupled else None
         if e.coupled:
             e.gd.build(e.D.pos, e.r_dem)
+        mark("grids")
         _lib.call("sphdem_surrogate_prep", e.params, _lib.sph_view(e.P), _lib.bnd_view(e.B),
                   dem, e.gs.struct(), e.gb.struct(), e.gd.struct() if e.coupled else None,
                   int(step), e.ws.struct(), st)
+        mark("prep")
         if e.coupled:
             ws = e.ws.struct()
             ws.flags = self._flags.data_ptr()
             _lib.call("sphdem_dem_substeps", e.params, dem, _lib.dembnd_view(e.W), e.gd.struct(),
                       e.gw.struct(), SUBSTEPS, self.skin, _lib.fptr(self._dem_scratch), ws, st)
+        mark("dem")
         out = self.net.forward(e.params, e.P, e.gs, e.gb)
+        mark("network")
         _lib.call("sphdem_mpnn_apply", e.params, _lib.sph_view(e.P), e.gs.struct(),
                   _lib.fptr(out), self.T, st)
+        mark("apply")
+
+    def call_costs(self, reps: int = 20) -> dict:
+        """Per-call cost in ms: each stage of an eager call (HIP events on
+        the launch stream between the stages), a captured call replayed, and
+        ten plain substeps replayed, every repetition from the same state
+        (put back between them, outside the timed region)."""
+        saved = self._state()
+        k0 = self.eng.step_index
+
+        def restore():
+            for s, name, t in saved:
+                getattr(s, name).copy_(t)
+            self.eng.step_index = k0
+            torch.cuda.synchronize()
+        names = ("grids", "prep", "dem", "network", "apply")
+        acc = {n: 0.0 for n in names}
+        for r in range(reps):
+            restore()
+            evs = [torch.cuda.Event(enable_timing=True)]
+            evs[0].record()
+            marks = []
+
+            def mark(stage):
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record()
+                marks.append((stage, ev))
+            self._enqueue_surrogate(SUBSTEPS, mark)
+            torch.cuda.synchronize()
+            prev = evs[0]
+            for stage, ev in marks:
+                if r > 0:
+                    acc[stage] += prev.elapsed_time(ev)
+                prev = ev
+        out = {f"{n}_ms": round(acc[n] / max(reps - 1, 1), 4) for n in names}
+        out["eager_call_ms"] = round(sum(acc.values()) / max(reps - 1, 1), 4)
+
+        def replay_ms(fn):
+            tot = 0.0
+            for r in range(reps + 1):
+                restore()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                fn()
+                b.record()
+                torch.cuda.synchronize()
+                if r > 0:
+                    tot += a.elapsed_time(b)
+            return tot / reps
+        if self._graphs is not None:
+            out["graph_call_ms"] = round(replay_ms(lambda: self._graphs[False].replay()), 4)
+        out["plain_call_ms"] = round(replay_ms(lambda: self.eng.run(SUBSTEPS)), 4)
+        call = out.get("graph_call_ms", out["eager_call_ms"])
+        out["speedup_if_every_call_kept"] = round(out["plain_call_ms"] / call, 3)
+        restore()
+        self._flags.zero_()
+        return out
 
     def capture(self) -> None:
         """hipGraphs of a surrogate call with and without the Shepard refresh
@@ -870,6 +934,8 @@ def rollout_c1(net: MPNN, sim_time: float = 0.5, judge_every: int = 50,
            "velocity_err_mean": float(np.mean(st.velocity_err)) if st.velocity_err else None,
            "final_plain": d_p, "final_surrogate": d_s, "health_end": ro.health(),
            "health_plain_end": ro.health(plain.P)}
+    # after the timed runs and their figures: the per-call cost breakdown
+    out["call_costs"] = ro.call_costs()
     if log:
         log(f"[surrogate] rollout: {out}")
     return out
