@@ -230,8 +230,18 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // 16 partial sums of a subject are added across lanes once per group.  Here:
 // nsub staged 16-candidate blocks, buf/vbuf holding candidates relative to the
 // group centre as (x', y', z', |q'|^2) and V.
+// The C operand of tile T: |m|^2 of subjects 16T + 4(l/16) + 0..3, read from
+// the wave's LDS slot (one broadcast read per tile) instead of held in VGPRs.
+struct COp {
+    const float *m2;
+    int comp;
+    __device__ __forceinline__ f32x4 operator[](int T) const {
+        return *reinterpret_cast<const f32x4 *>(m2 + 16 * T + 4 * comp);
+    }
+};
+
 __device__ __forceinline__ void mfma_blocks(f32x4 (&acc)[4], const float (&a)[4],
-                                            const f32x4 (&c)[4], const V4<float> *buf,
+                                            const COp &c, const V4<float> *buf,
                                             const float *vbuf, int nsub, int lane, float k1,
                                             float k2) {
     const int col = lane & 15, comp = lane >> 4;
@@ -312,7 +322,7 @@ __device__ __forceinline__ float quarter_sum(float v) {
 // (Issuing the next block's products before this block's terms, with one
 // set of results or two, measured slower.)
 __device__ __forceinline__ void mfma_blocks_sym(f32x4 (&acc)[4], const float (&a)[4],
-                                                const f32x4 (&c)[4], const V4<float> *buf,
+                                                const COp &c, const V4<float> *buf,
                                                 const float *vbuf, const int *sid, int nsub,
                                                 int lane, float k1, float k2, const float *vsub,
                                                 const SymCtx &sym) {
@@ -356,6 +366,7 @@ struct SymQueue {
     int *sid;
     float *vsub;   // [64] V of the wave's subjects by lane (0 outside the group)
     int fill;
+    float *msub;   // [64] |m|^2 of the wave's subjects by lane (the products' C operand)
 };
 
 // FAST, FP32, culled: a candidate farther than the support from the box of
@@ -368,7 +379,7 @@ struct SymQueue {
 // candidate within the support is dropped.
 template <bool SYM>
 __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&a)[4],
-                                                 const f32x4 (&c)[4], int &fill, int a0, int b0,
+                                                 const COp &c, int &fill, int a0, int b0,
                                                  const V4<float> *pos, const float *Vs,
                                                  V4<float> *buf, float *vbuf, int lane,
                                                  const V3<float> &gc, const V3<float> &he,
@@ -459,7 +470,7 @@ __device__ __forceinline__ void queue_range_mfma(f32x4 (&acc)[4], const float (&
 // The symmetric queue's last entries, padded to a multiple of 16 with
 // candidates 1e15 away (V = 0, no slot).
 __device__ __forceinline__ void flush_queue_sym(f32x4 (&acc)[4], const float (&a)[4],
-                                                const f32x4 (&c)[4], SymQueue &sq, int lane,
+                                                const COp &c, SymQueue &sq, int lane,
                                                 float k1, float k2, const SymCtx &sym) {
     if (sq.fill == 0) return;
     const int nsub = (sq.fill + 15) >> 4;
@@ -478,7 +489,7 @@ __device__ __forceinline__ void flush_queue_sym(f32x4 (&acc)[4], const float (&a
 // The queue's last entries, padded to a multiple of 16 with candidates
 // 1e15 away (V = 0), which add 0.
 __device__ __forceinline__ void flush_queue_mfma(f32x4 (&acc)[4], const float (&a)[4],
-                                                 const f32x4 (&c)[4], int &fill,
+                                                 const COp &c, int &fill,
                                                  V4<float> *buf, float *vbuf, int lane,
                                                  float k1, float k2) {
     if (fill == 0) return;
@@ -579,7 +590,7 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
     static_assert(!SYM || kMfma, "the symmetric sums are an FP32 fast-path form");
     const int col = lane & 15, comp = lane >> 4;
     float ma[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    f32x4 mc[4];
+    const COp mc{sq.msub, comp};
     f32x4 acc[4] = {};
     int fill = 0;
     if constexpr (SYM) {
@@ -594,9 +605,9 @@ __device__ bool stream_group(T &flt, const V4<T> &me, bool in, const GridV<T> &g
                         mzs = __shfl((float)m.z, sl, 64);
             ma[Tt] = comp == 0 ? -2.0f * mxs : comp == 1 ? -2.0f * mys
                                                : comp == 2 ? -2.0f * mzs : 1.0f;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) mc[Tt][t] = __shfl((float)m2, 16 * Tt + 4 * comp + t, 64);
         }
+        sq.msub[lane] = (float)m2;
+        __builtin_amdgcn_wave_barrier();
     }
     if constexpr (kMfma) {
         // FP32 FAST: the lanes find the slot ranges of 64 bucket rows at once
@@ -795,7 +806,11 @@ struct ChunkV {
 };
 
 template <typename T, bool FAST, bool SYM, bool CHUNKS>
-__global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
+// At least 3 waves per SIMD: 168 VGPRs; the FP32 fast kernels then spill a
+// few registers in their per-item and per-group setup, none in the
+// candidate and product loops (measured at c2: 14.60 -> 12.54 ms against 2
+// waves per SIMD; 4 waves, 128 VGPRs, spill in the loops: 14.73 ms).
+__global__ __launch_bounds__(256, 3) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T> D, GridV<T> gs,
                                                     GridV<T> gd, const T *Vs, const int *flag,
                                                     T csup, T hu, T thr2, int xt, int remap,
                                                     unsigned long long *col_sum,
@@ -814,8 +829,9 @@ __global__ __launch_bounds__(256) void k_por_stream(Prm<T> p, SphV<T> P, DemV<T>
     const int wv = threadIdx.x >> 6;
     V4<T> *wbuf = stage + wv * kSlot;
     T *vbuf = vstage + wv * kSlot;
+    __shared__ __align__(16) float msubstage[4 * 64];
     const SymQueue sq{sstage + wv * kSymSlot, svstage + wv * kSymSlot, sidstage + wv * kSymSlot,
-                      vsubstage + wv * (SYM ? 64 : 1), 0};
+                      vsubstage + wv * (SYM ? 64 : 1), 0, msubstage + wv * 64};
     const int wave = subject_id(remap) >> 6;
     if constexpr (CHUNKS) {
         // items (chunk, part), the parts of a chunk on neighbouring waves
