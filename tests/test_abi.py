@@ -76,3 +76,22 @@ def test_integration_grid_binding_matches_the_header():
     assert ctypes.sizeof(Grid) == ctypes.sizeof(_lib.GridStruct)
     for (name, _), (name2, _) in zip(Grid._fields_, _lib.GridStruct._fields_):
         assert getattr(Grid, name).offset == getattr(_lib.GridStruct, name2).offset, name
+
+
+def test_tuning_round_trips_and_rejects_out_of_range_values():
+    """Launch shapes are host-side state: set, read back, and invalid
+    values refused with the library's error (no device work)."""
+    saved = _lib.get_tuning()
+    try:
+        _lib.set_tuning({"por_parts": 7, "por_x_tile": 3, "dem_beside": 0,
+                         "block": {"porosity": 64}})
+        t = _lib.get_tuning()
+        assert (t["por_parts"], t["por_x_tile"], t["dem_beside"]) == (7, 3, 0)
+        assert t["block"]["porosity"] == 64
+        for bad in ({"por_parts": 0}, {"por_parts": 17}, {"por_x_tile": 7},
+                    {"dem_beside": 2}, {"block": {"porosity": 96}}):
+            with pytest.raises(ValueError, match="sphdem_tuning_set"):
+                _lib.set_tuning(bad)
+        assert _lib.get_tuning() == t
+    finally:
+        _lib.set_tuning(saved)

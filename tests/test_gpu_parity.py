@@ -279,6 +279,43 @@ def test_coupling_kernels(uniform_h, math_mode):
     S.compare(["pos", "vel"])
 
 
+@pytest.mark.parametrize("grid_slice", [None, 10])
+@pytest.mark.parametrize("form", ["parts1", "parts3", "parts8", "no_list", "one_sided"])
+def test_fast_porosity_forms(grid_slice, form):
+    """Every form of the fast FP32 porosity against the oracle: the work list
+    of per-row chunks with 1, 3 or 8 waves per chunk, 64 consecutive slots per
+    wave (no list), and one sum per subject (no symmetric sums), on the
+    scene's grid and on one folded to 10 buckets per axis (some groups then
+    walk their queries per subject and apply the pair rule pair by pair)."""
+    from warp_sph_gpt_amd import _lib
+    S = Scene() if grid_slice is None else Scene(grid_slice=grid_slice)
+    s = S.solv
+    (P, B, D, W), (gP, gB, gD, gW) = S.o, S.g
+    rmax, rbmax, hmax, hu = S.scales
+    ws = K.workspace(DEV, torch.float32, gP.n, gB.n, gD.n)
+    saved = (ws.fl_chunk, ws.fl_sum, _lib.get_tuning(), _lib.fast_math())
+    try:
+        _lib.set_fast_math(True)
+        if form.startswith("parts"):
+            _lib.set_tuning({"por_parts": int(form[5:])})
+        elif form == "no_list":
+            ws.fl_chunk = None
+        else:
+            ws.fl_sum = None
+        O.Kernel_prep_sphdem(P, B, D, *S.og[:3], 2 * hmax, hu, s.support, s.h,
+                             s.dem_porosity_min, s.dem_porosity_max)
+        K.Kernel_prep_sphdem(gP, gB, gD, *S.gg[:3], 2 * hmax, hu, s.support, s.h,
+                             s.dem_porosity_min, s.dem_porosity_max)
+        S.compare(["porosity"])
+        assert P.porosity.min() < 0.99
+        # the fixed-point accumulators are left cleared for the next step
+        assert int(ws.fl_list.abs().sum()) == 0
+    finally:
+        ws.fl_chunk, ws.fl_sum = saved[0], saved[1]
+        _lib.set_tuning(saved[2])
+        _lib.set_fast_math(saved[3])
+
+
 @pytest.mark.parametrize("grid_slice", [12, 10, 8])
 def test_porosity_on_folded_grids(grid_slice, math_mode):
     """The coupling-scale porosity with the tank folded onto a few buckets per

@@ -1,4 +1,4 @@
-"""Surrogate rollout machinery on the GPU: the one-workgroup DEM substeps
+"""Surrogate rollout machinery on the GPU: the DEM substeps (a wave per DEM)
 against the oracle's plain DEM stage (contacts, CSR history, integration)
 with the coupling force held; the judged call keeps the plain result and a
 rejected call reverts to the last judged state and redoes it with the plain
