@@ -5,11 +5,12 @@
 //
 // Build of n points into dim_x*dim_y*dim_z buckets:
 //   1. key:     point_cells[i] = bucket(int(p*inv)); rank[i] = atomicAdd(count[key])
-//   2. scan:    cell_lo = exclusive_scan(count)
+//   2. scan:    cell_lo = exclusive_scan(count), cell_hi = cell_lo + count
+//               (the scan's last pass)
 //   3. scatter: point_ids[cell_lo[key] + rank[i]] = i
-//   4. fix:     cell_hi = cell_lo + count; each bucket's ids sorted ascending,
-//               which makes the order equal to a stable sort by key and
-//               independent of the order the atomics were served in
+//   4. order:   each bucket's ids sorted ascending, which makes the order
+//               equal to a stable sort by key and independent of the order
+//               the atomics were served in
 //   5. sorted:  sorted[k] = (p[id], w[id]) for coalesced neighbour reads
 #include "common.hpp"
 
@@ -93,21 +94,33 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// Exclusive scan of one tile of `in` into `out`, offset by `carry`; returns
-// the tile total to every thread.
-__device__ int scan_tile(const int *in, int *out, int n, int tile_base, int carry) {
-    __shared__ int wsum[kScanThreads / 64];
+// Exclusive scan of one tile (THREADS x kScanItems values) of `in` into
+// `out`, offset by `carry`; returns the tile total to every thread.  With
+// `vec` (both arrays 16-byte aligned) a thread's 8 values move as two int4
+// when all of them are in range.
+template <int THREADS>
+__device__ int scan_tile(const int *in, int *out, int n, int tile_base, int carry, bool vec) {
+    __shared__ int wsum[THREADS / 64];
     __shared__ int total;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int base = tile_base + t * kScanItems;
+    const bool whole = vec && base + kScanItems <= n;
+    int x[kScanItems];
+    if (whole) {
+        const int4 a = *reinterpret_cast<const int4 *>(in + base);
+        const int4 b = *reinterpret_cast<const int4 *>(in + base + 4);
+        x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+        x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) x[k] = base + k < n ? in[base + k] : 0;
+    }
     int v[kScanItems];
     int tsum = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const int idx = base + k;
-        const int x = idx < n ? in[idx] : 0;
         v[k] = tsum;
-        tsum += x;
+        tsum += x[k];
     }
     int incl = tsum;
 #pragma unroll
@@ -120,12 +133,17 @@ __device__ int scan_tile(const int *in, int *out, int n, int tile_base, int carr
     int wofs = 0;
     for (int w = 0; w < wid; ++w) wofs += wsum[w];
     const int excl = carry + wofs + incl - tsum;
+    if (whole) {
+        *reinterpret_cast<int4 *>(out + base) =
+            int4{v[0] + excl, v[1] + excl, v[2] + excl, v[3] + excl};
+        *reinterpret_cast<int4 *>(out + base + 4) =
+            int4{v[4] + excl, v[5] + excl, v[6] + excl, v[7] + excl};
+    } else {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int idx = base + k;
-        if (idx < n) out[idx] = v[k] + excl;
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < n) out[base + k] = v[k] + excl;
     }
-    if (t == kScanThreads - 1) total = wofs + incl;
+    if (t == THREADS - 1) total = wofs + incl;
     __syncthreads();
     const int r = total;
     __syncthreads();
@@ -133,21 +151,42 @@ __device__ int scan_tile(const int *in, int *out, int n, int tile_base, int carr
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const int *in, int *out, int n,
-                                                             int *partial) {
-    const int tot = scan_tile(in, out, n, blockIdx.x * kScanTile, 0);
+                                                             int *partial, bool vec) {
+    const int tot = scan_tile<kScanThreads>(in, out, n, blockIdx.x * kScanTile, 0, vec);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// One block scans the tile totals in place (any count, chunk by chunk).
-__global__ __launch_bounds__(kScanThreads) void k_scan_partials(int *partial, int tiles) {
+// One block scans the tile totals in place (8,192 per pass, any count).
+constexpr int kPartialThreads = 1024;
+__global__ __launch_bounds__(kPartialThreads) void k_scan_partials(int *partial, int tiles,
+                                                                   bool vec) {
     int carry = 0;
-    for (int base = 0; base < tiles; base += kScanTile)
-        carry += scan_tile(partial, partial, tiles, base, carry);
+    for (int base = 0; base < tiles; base += kPartialThreads * kScanItems)
+        carry += scan_tile<kPartialThreads>(partial, partial, tiles, base, carry, vec);
 }
 
-__global__ void k_scan_add(int *out, int n, const int *partial) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n) out[idx] += partial[idx / kScanTile];
+// out += the tile offsets, four values per thread; with `hi` (the counts on
+// entry) also hi = out + count, the buckets' ends.
+__global__ void k_scan_add(int *out, int n, const int *partial, bool add, int *hi, bool vec) {
+    const int i0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= n) return;
+    const int p = add ? partial[i0 / kScanTile] : 0;
+    if (vec && i0 + 4 <= n) {
+        int4 o = *reinterpret_cast<const int4 *>(out + i0);
+        o.x += p; o.y += p; o.z += p; o.w += p;
+        *reinterpret_cast<int4 *>(out + i0) = o;
+        if (hi) {
+            int4 c = *reinterpret_cast<const int4 *>(hi + i0);
+            c.x += o.x; c.y += o.y; c.z += o.z; c.w += o.w;
+            *reinterpret_cast<int4 *>(hi + i0) = c;
+        }
+    } else {
+        for (int i = i0; i < n && i < i0 + 4; ++i) {
+            const int o = out[i] + p;
+            out[i] = o;
+            if (hi) hi[i] += o;
+        }
+    }
 }
 
 int scan_scratch(int n) { return blocks_for(n, kScanTile) + 1; }
@@ -182,15 +221,27 @@ int noalloc_scan_scratch(int **out) {
     return SPHDEM_OK;
 }
 
-int exclusive_scan(const int *in, int *out, int n, int *scratch, hipStream_t st) {
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// The scan; with `hi` (holding the counts that `in` also points to) the
+// last pass writes the bucket ends hi = out + count as well.
+static int scan_impl(const int *in, int *out, int n, int *scratch, hipStream_t st, int *hi) {
     if (n <= 0) return SPHDEM_OK;
     const int tiles = blocks_for(n, kScanTile);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, st, in, out, n, scratch);
-    if (tiles > 1) {
-        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, st, scratch, tiles);
-        hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 256)), dim3(256), 0, st, out, n, scratch);
-    }
+    const bool vec = aligned16(in) && aligned16(out) && (!hi || aligned16(hi));
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, st, in, out, n, scratch,
+                       vec);
+    if (tiles > 1)
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kPartialThreads), 0, st, scratch, tiles,
+                           aligned16(scratch));
+    if (tiles > 1 || hi)
+        hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, 4 * 256)), dim3(256), 0, st, out, n,
+                           (const int *)scratch, tiles > 1, hi, vec);
     return check_launch("exclusive_scan");
+}
+
+int exclusive_scan(const int *in, int *out, int n, int *scratch, hipStream_t st) {
+    return scan_impl(in, out, n, scratch, st, nullptr);
 }
 
 // ---------------------------------------------------------------- grid build
@@ -238,12 +289,12 @@ __global__ void k_grid_scatter(const int *cells, const int *rank, const int *lo,
 
 // Warp's order inside a bucket is ascending point index.  The scatter left
 // each bucket's ids in atomic-arrival order in `tmp`; point i takes the slot
-// after the ids of its bucket smaller than i (cell_hi still holds counts).
-__global__ void k_grid_order(const int *cells, const int *lo, const int *cnt, const int *tmp,
+// after the ids of its bucket smaller than i.
+__global__ void k_grid_order(const int *cells, const int *lo, const int *hi, const int *tmp,
                              int n, int *ids) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int c = cells[i], a = lo[c], e = a + cnt[c];
+    const int c = cells[i], a = lo[c], e = hi[c];
     int pos = a, k = a;
     // eight independent loads in flight, not one load and wait per slot
     for (; k + 8 <= e; k += 8) {
@@ -255,12 +306,6 @@ __global__ void k_grid_order(const int *cells, const int *lo, const int *cnt, co
     }
     for (; k < e; ++k) pos += tmp[k] < i;
     ids[pos] = i;
-}
-
-// cell_hi holds the counts on entry and the bucket ends on exit.
-__global__ void k_grid_fix(const int *lo, int *hi, int ncell) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncell) hi[c] = lo[c] + hi[c];
 }
 
 template <typename T>
@@ -286,7 +331,9 @@ static int grid_build(sphdem_grid_t *g, const void *points, int n, double radius
         hipLaunchKernelGGL(k_grid_key<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            (const T *)points, n, inv, g->dim_x, g->dim_y, g->dim_z, inst,
                            g->point_cells, g->rank, g->cell_hi);
-    int rc = exclusive_scan(g->cell_hi, g->cell_lo, ncell, g->scan_tmp, st);
+    // cell_lo = exclusive scan of the counts in cell_hi, which become the
+    // bucket ends cell_lo + count in the same pass
+    int rc = scan_impl(g->cell_hi, g->cell_lo, ncell, g->scan_tmp, st, g->cell_hi);
     if (rc) return rc;
     if (n > 0) {
         // the sorted copy's storage (4n values, written last) holds the
@@ -297,8 +344,6 @@ static int grid_build(sphdem_grid_t *g, const void *points, int n, double radius
         hipLaunchKernelGGL(k_grid_order, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            g->point_cells, g->cell_lo, g->cell_hi, tmp, n, g->point_ids);
     }
-    hipLaunchKernelGGL(k_grid_fix, dim3(blocks_for(ncell, bt)), dim3(bt), 0, st, g->cell_lo,
-                       g->cell_hi, ncell);
     if (n > 0)
         hipLaunchKernelGGL(k_grid_sorted<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            (const T *)points, (const T *)w, g->point_ids, n, (V4<T> *)g->sorted);
