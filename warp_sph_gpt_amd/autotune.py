@@ -5,6 +5,8 @@ sub-group width; see sphdem_tuning_t) decide only which lanes evaluate which
 subjects, so every setting gives the same bits and the tuner may time them
 on the live simulation.  For each candidate it enqueues a few real steps
 with per-pass HIP-event timing and keeps, pass by pass, the fastest shape.
+The porosity's waves per work item (fast FP32 path) regroup its fixed-point
+partial sums, within that path's tolerance; they are timed the same way.
 
     shapes = autotune(engine.step, steps=3)
 """
@@ -20,6 +22,7 @@ from . import _lib
 BLOCKS = (64, 128, 256)
 REMAPS = (0, 1)
 X_TILES = (3, 4, 5)
+POR_PARTS = (2, 4, 8)
 
 
 def _time_whole_steps(step_fn, steps: int) -> float:
@@ -78,6 +81,13 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
             xt_ms[xt] = _time_steps(step_fn, steps).get("porosity", 0.0)
         cfg["por_x_tile"] = min(xt_ms, key=xt_ms.get)
         _lib.set_tuning({"por_x_tile": cfg["por_x_tile"]})
+    parts_ms = {}
+    if "porosity" in passes:
+        for pp in POR_PARTS:
+            _lib.set_tuning({"por_parts": pp})
+            parts_ms[pp] = _time_steps(step_fn, steps).get("porosity", 0.0)
+        cfg["por_parts"] = min(parts_ms, key=parts_ms.get)
+        _lib.set_tuning({"por_parts": cfg["por_parts"]})
     # where the DEM interpolation runs changes only the overlap: whole steps
     beside_ms = {}
     if "porosity" in passes:
@@ -90,6 +100,7 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
             log(f"[autotune] step ms with the DEM interpolation beside / after the porosity: "
                 f"{beside_ms[1] / steps:.3f} / {beside_ms[0] / steps:.3f}")
     out = {"tuning": cfg, "ms": table, "por_x_tile_ms": {str(k): v for k, v in xt_ms.items()},
+           "por_parts_ms": {str(k): v for k, v in parts_ms.items()},
            "dem_beside_ms": {str(k): v for k, v in beside_ms.items()},
            "steps_per_candidate": steps}
     if log:
