@@ -118,9 +118,9 @@ typedef struct {
  * Sizes: fluid arrays >= fluid count, wall arrays >= wall count. */
 typedef struct {
     void *fl_rho, *fl_pres, *fl_V;  /* [nf] density, pressure, m/rho */
-    void *fl_vel, *fl_pgf;          /* [nf*4] (vel, 0), (-grad p, rho) */
+    void *fl_vel, *fl_pgf;          /* [nf*4] (vel, rho), (-grad p, rho) */
     void *wl_rho, *wl_pres;         /* [nb] */
-    void *wl_vel;                   /* [nb*4] */
+    void *wl_vel;                   /* [nb*4] (vel, rho) */
     int *scan_tmp;                  /* [sphdem_scan_scratch(n_dem + 1)] */
     int *flags;                     /* [4] bit 0: DEM CSR overflow */
     int *fl_flag;                   /* [nf] near-DEM flags */

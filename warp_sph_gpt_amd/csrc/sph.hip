@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
             w.fl_rho[k] = r;
             w.fl_pres[k] = pr;
             P.pres[i] = pr;
-            w.fl_vel[k] = V4<T>{P.vel[3 * i], P.vel[3 * i + 1], P.vel[3 * i + 2], (T)0};
+            w.fl_vel[k] = V4<T>{P.vel[3 * i], P.vel[3 * i + 1], P.vel[3 * i + 2], r};
         } else {
             P.rho[i] = r;
         }
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kThreads) void k_wall_density(Prm<T> p, SphV<T> P, 
         B.pres[bi] = pr;
         w.wl_rho[k] = rho;
         w.wl_pres[k] = pr;
-        w.wl_vel[k] = V4<T>{B.vel[3 * bi], B.vel[3 * bi + 1], B.vel[3 * bi + 2], (T)0};
+        w.wl_vel[k] = V4<T>{B.vel[3 * bi], B.vel[3 * bi + 1], B.vel[3 * bi + 2], rho};
     }
 }
 
@@ -193,7 +193,9 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
     const FastDW<T> fdw(h);
     const WendlandK<T> wk(h);
     V3<T> acc = mk3<T>(0, 0, 0), pgf = mk3<T>(0, 0, 0);
-    auto body = [&](const GridV<T> &g, const T *nrho, const T *npres, const V4<T> *nvel) {
+    // neighbours' (velocity, density) come in one load from the sorted
+    // (vel, rho) records the density passes write
+    auto body = [&](const GridV<T> &g, const T *npres, const V4<T> *nvel) {
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
             each_slot<4>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int s, const V4<T> &q) {
                 const V3<T> rij = mk3(me.x - q.x, me.y - q.y, me.z - q.z);
@@ -202,24 +204,26 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
                     if (td > r2min && td < ss) {
                         const T gr = fdw.over_r(td);  // dW/dr / r
                         const V3<T> dwij = gr * rij;
-                        const T rhoj = nrho[s], presj = npres[s];
+                        const V4<T> vr = nvel[s];   // (velocity, density)
+                        const T rhoj = vr.w, presj = npres[s];
                         const T irho = fast_rcp(rhoj);
                         if (ACC) {
                             acc = acc - (q.w * (ai + presj * irho * irho)) * dwij;
                             const T visc = two_mu * gr * td * fast_rcp(rhoi * rhoj * (td + reg));
-                            acc = acc + (q.w * visc) * (vi - xyz(nvel[s]));
+                            acc = acc + (q.w * visc) * (vi - xyz(vr));
                         }
                         if (PGF) pgf = pgf - ((q.w * irho) * (presj - presi)) * dwij;
                     }
                 } else if (td > r2min && td < ss) {
                     const T r = msqrt(td);
                     const T dwv = wendland_dw(r, wk);
-                    const T rhoj = nrho[s], presj = npres[s];
+                    const V4<T> vr = nvel[s];   // (velocity, density)
+                    const T rhoj = vr.w, presj = npres[s];
                     if (ACC) {
                         const V3<T> dwij = dwv * (rij / r);
                         acc = acc - (q.w * (ai + presj / (rhoj * rhoj))) * dwij;
                         const T visc = two_mu * dot(rij, dwij) / (rhoi * rhoj * (td + reg));
-                        acc = acc + (q.w * visc) * (vi - xyz(nvel[s]));
+                        acc = acc + (q.w * visc) * (vi - xyz(vr));
                     }
                     if (PGF) {
                         const V3<T> dwp = (dwv * rij) / r;
@@ -229,8 +233,8 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
             });
         });
     };
-    body(gs, w.fl_rho, w.fl_pres, w.fl_vel);
-    body(gb, w.wl_rho, w.wl_pres, w.wl_vel);
+    body(gs, w.fl_pres, w.fl_vel);
+    body(gb, w.wl_pres, w.wl_vel);
     if (ACC) st3(P.acc, i, acc + mk3<T>(0, 0, -p.g));
     if (PUBLISH) P.rho[i] = rhoi;
     if (PGF) {
@@ -250,7 +254,7 @@ __global__ void k_gather_fluid(SphV<T> P, GridV<T> gs, WorkV<T> w) {
     const T r = P.rho[i];
     w.fl_rho[k] = r;
     w.fl_pres[k] = P.pres[i];
-    w.fl_vel[k] = V4<T>{P.vel[3 * i], P.vel[3 * i + 1], P.vel[3 * i + 2], (T)0};
+    w.fl_vel[k] = V4<T>{P.vel[3 * i], P.vel[3 * i + 1], P.vel[3 * i + 2], r};
     w.fl_V[k] = P.m[i] / r;
     w.fl_pgf[k] = V4<T>{P.pgf[3 * i], P.pgf[3 * i + 1], P.pgf[3 * i + 2], r};
 }
@@ -262,7 +266,7 @@ __global__ void k_gather_wall(BndV<T> B, GridV<T> gb, WorkV<T> w) {
     const int bi = gb.ids[k];
     w.wl_rho[k] = B.rho[bi];
     w.wl_pres[k] = B.pres[bi];
-    w.wl_vel[k] = V4<T>{B.vel[3 * bi], B.vel[3 * bi + 1], B.vel[3 * bi + 2], (T)0};
+    w.wl_vel[k] = V4<T>{B.vel[3 * bi], B.vel[3 * bi + 1], B.vel[3 * bi + 2], B.rho[bi]};
 }
 
 // KERNEL_step.py:6-22 (tid order; elementwise).
