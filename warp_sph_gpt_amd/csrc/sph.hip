@@ -172,8 +172,9 @@ __global__ void k_pres(Prm<T> p, const T *rho, T *pres, int n) {
 //  PUBLISH: write the sorted density back to P.rho (fused step).
 //  PGF also leaves (pgf, rho) and m/rho in sorted order for the coupling.
 //  FAST: dW/dr * rij / r is c' u^3 rij with u = 1 - r/2h (FastDW: one
-//  hardware square root, no division by r), and the divisions by rho_j and
-//  by rho_i rho_j (r^2 + 0.01 h^2) are hardware reciprocals.
+//  hardware square root, no division by r), the divisions by rho_j and
+//  by rho_i rho_j (r^2 + 0.01 h^2) are hardware reciprocals, and r^2 and
+//  the sums are formed with fused multiply-adds.
 template <typename T, bool ACC, bool PGF, bool PUBLISH, bool FAST>
 __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, BndV<T> B,
                                                           GridV<T> gs, GridV<T> gb, WorkV<T> w,
@@ -199,22 +200,37 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
         query_runs(g, me.x, me.y, me.z, sup, [&](int a, int b) {
             each_slot<4>(a, b, 1, [&](int s) { return g.s[s]; }, [&](int s, const V4<T> &q) {
                 const V3<T> rij = mk3(me.x - q.x, me.y - q.y, me.z - q.z);
-                const T td = dot(rij, rij);
                 if constexpr (FAST) {
+                    // fused multiply-adds throughout; grad W = gr rij enters
+                    // each sum as one scalar times rij
+                    const T td = fma(rij.z, rij.z, fma(rij.y, rij.y, rij.x * rij.x));
                     if (td > r2min && td < ss) {
                         const T gr = fdw.over_r(td);  // dW/dr / r
-                        const V3<T> dwij = gr * rij;
                         const V4<T> vr = nvel[s];   // (velocity, density)
                         const T rhoj = vr.w, presj = npres[s];
                         const T irho = fast_rcp(rhoj);
                         if (ACC) {
-                            acc = acc - (q.w * (ai + presj * irho * irho)) * dwij;
-                            const T visc = two_mu * gr * td * fast_rcp(rhoi * rhoj * (td + reg));
-                            acc = acc + (q.w * visc) * (vi - xyz(vr));
+                            const T a = -(q.w * fma(presj * irho, irho, ai)) * gr;
+                            acc.x = fma(a, rij.x, acc.x);
+                            acc.y = fma(a, rij.y, acc.y);
+                            acc.z = fma(a, rij.z, acc.z);
+                            const T b = q.w * (two_mu * gr * td) *
+                                        fast_rcp(rhoi * rhoj * (td + reg));
+                            acc.x = fma(b, vi.x - vr.x, acc.x);
+                            acc.y = fma(b, vi.y - vr.y, acc.y);
+                            acc.z = fma(b, vi.z - vr.z, acc.z);
                         }
-                        if (PGF) pgf = pgf - ((q.w * irho) * (presj - presi)) * dwij;
+                        if (PGF) {
+                            const T c = -((q.w * irho) * (presj - presi)) * gr;
+                            pgf.x = fma(c, rij.x, pgf.x);
+                            pgf.y = fma(c, rij.y, pgf.y);
+                            pgf.z = fma(c, rij.z, pgf.z);
+                        }
                     }
-                } else if (td > r2min && td < ss) {
+                    return;
+                }
+                const T td = dot(rij, rij);
+                if (td > r2min && td < ss) {
                     const T r = msqrt(td);
                     const T dwv = wendland_dw(r, wk);
                     const V4<T> vr = nvel[s];   // (velocity, density)
