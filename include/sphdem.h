@@ -158,7 +158,8 @@ typedef struct {
     void *fl_sum;
     /* [7 * cap_f] ints or NULL: scratch of the porosity's work list (the
      * near-DEM slots of each bucket row, compacted and cut every 64; one
-     * wavefront per entry; counts in fl_count[0..1]), rebuilt every step.
+     * wavefront per entry; counts in fl_count[0..1], the stream's item
+     * counter in fl_count[2]), rebuilt every step.
      * NULL: one wavefront per 64 consecutive slots (the same sums; with the
      * reference's arithmetic the same bits). */
     int *fl_chunk;
@@ -344,13 +345,19 @@ int sphdem_profile_read(double *ms, int *counts, int n);
  *                    por_parts-th bucket row of the item's query: 1..16
  *                    (default 4).  The terms are the same; their grouping
  *                    into fixed-point partial sums changes with it, so the
- *                    sums may differ in the last units of that form. */
+ *                    sums may differ in the last units of that form.
+ *   por_queue        how the porosity's waves take their (item, part) work
+ *                    (work list only): 0 wave w takes items w, w + waves,
+ *                    ...; 1 every wave takes the next item from one counter
+ *                    when it finishes one (default 1).  The same partial
+ *                    sums under both values. */
 typedef struct {
     int block[SPHDEM_NUM_PASSES];
     int xcd_remap[SPHDEM_NUM_PASSES];
     int por_x_tile;
     int dem_beside;
     int por_parts;
+    int por_queue;
 } sphdem_tuning_t;
 int sphdem_tuning_get(sphdem_tuning_t *t);
 int sphdem_tuning_set(const sphdem_tuning_t *t);

@@ -280,13 +280,16 @@ def test_coupling_kernels(uniform_h, math_mode):
 
 
 @pytest.mark.parametrize("grid_slice", [None, 10])
-@pytest.mark.parametrize("form", ["parts1", "parts3", "parts8", "no_list", "one_sided"])
+@pytest.mark.parametrize("form", ["parts1", "parts3", "parts8", "queue0", "no_list",
+                                  "one_sided"])
 def test_fast_porosity_forms(grid_slice, form):
     """Every form of the fast FP32 porosity against the oracle: the work list
-    of per-row chunks with 1, 3 or 8 waves per chunk, 64 consecutive slots per
-    wave (no list), and one sum per subject (no symmetric sums), on the
-    scene's grid and on one folded to 10 buckets per axis (some groups then
-    walk their queries per subject and apply the pair rule pair by pair)."""
+    of per-row chunks with 1, 3 or 8 waves per chunk, its items taken by
+    wave index (queue 0) or from one counter (the default), 64 consecutive
+    slots per wave (no list), and one sum per
+    subject (no symmetric sums), on the scene's grid and on one folded to 10
+    buckets per axis (some groups then walk their queries per subject and
+    apply the pair rule pair by pair)."""
     from warp_sph_gpt_amd import _lib
     S = Scene() if grid_slice is None else Scene(grid_slice=grid_slice)
     s = S.solv
@@ -298,6 +301,8 @@ def test_fast_porosity_forms(grid_slice, form):
         _lib.set_fast_math(True)
         if form.startswith("parts"):
             _lib.set_tuning({"por_parts": int(form[5:])})
+        elif form.startswith("queue"):
+            _lib.set_tuning({"por_queue": int(form[5:])})
         elif form == "no_list":
             ws.fl_chunk = None
         else:
@@ -314,6 +319,34 @@ def test_fast_porosity_forms(grid_slice, form):
         ws.fl_chunk, ws.fl_sum = saved[0], saved[1]
         _lib.set_tuning(saved[2])
         _lib.set_fast_math(saved[3])
+
+
+def test_porosity_item_queues_give_the_same_bits():
+    """How the waves take the work items (by wave index or from one counter)
+    changes which wave evaluates an item, not its partial sums: the porosity
+    and the unscaled sums are the same bits."""
+    from warp_sph_gpt_amd import _lib
+    S = Scene()
+    s = S.solv
+    gP, gB, gD, gW = S.g
+    rmax, rbmax, hmax, hu = S.scales
+    ws = K.workspace(DEV, torch.float32, gP.n, gB.n, gD.n)
+    saved = (_lib.get_tuning(), _lib.fast_math())
+    out = {}
+    try:
+        _lib.set_fast_math(True)
+        for q in (0, 1):
+            _lib.set_tuning({"por_queue": q, "por_parts": 3})
+            K.Kernel_prep_sphdem(gP, gB, gD, *S.gg[:3], 2 * hmax, hu, s.support, s.h,
+                                 s.dem_porosity_min, s.dem_porosity_max)
+            torch.cuda.synchronize()
+            out[q] = (np_(gP.porosity).copy(), np_(ws.fl_sum)[:gP.n].copy())
+    finally:
+        _lib.set_tuning(saved[0])
+        _lib.set_fast_math(saved[1])
+    assert (out[0][0] < 0.99).any()
+    np.testing.assert_array_equal(out[1][0], out[0][0], err_msg="porosity")
+    np.testing.assert_array_equal(out[1][1], out[0][1], err_msg="unscaled sums")
 
 
 @pytest.mark.parametrize("grid_slice", [12, 10, 8])

@@ -6,7 +6,8 @@ subjects, so every setting gives the same bits and the tuner may time them
 on the live simulation.  For each candidate it enqueues a few real steps
 with per-pass HIP-event timing and keeps, pass by pass, the fastest shape.
 The porosity's waves per work item (fast FP32 path) regroup its fixed-point
-partial sums, within that path's tolerance; they are timed the same way.
+partial sums, within that path's tolerance; they are timed the same way,
+together with the order in which the waves take the work items.
 
     shapes = autotune(engine.step, steps=3)
 """
@@ -23,6 +24,7 @@ BLOCKS = (64, 128, 256)
 REMAPS = (0, 1)
 X_TILES = (3, 4, 5)
 POR_PARTS = (2, 4, 8)
+POR_QUEUES = (0, 1)
 
 
 def _time_whole_steps(step_fn, steps: int) -> float:
@@ -83,11 +85,21 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
         _lib.set_tuning({"por_x_tile": cfg["por_x_tile"]})
     parts_ms = {}
     if "porosity" in passes:
-        for pp in POR_PARTS:
-            _lib.set_tuning({"por_parts": pp})
-            parts_ms[pp] = _time_steps(step_fn, steps).get("porosity", 0.0)
-        cfg["por_parts"] = min(parts_ms, key=parts_ms.get)
-        _lib.set_tuning({"por_parts": cfg["por_parts"]})
+        # waves per work item, how the waves take the items and (which only
+        # matters when they take them by wave index) the tile order, jointly
+        for q in POR_QUEUES:
+            for r in (REMAPS if q == 0 else (cfg["xcd_remap"]["porosity"],)):
+                for pp in POR_PARTS:
+                    _lib.set_tuning({"por_parts": pp, "por_queue": q,
+                                     "xcd_remap": {"porosity": r}})
+                    parts_ms[(q, r, pp)] = _time_steps(step_fn, steps).get("porosity", 0.0)
+        q, r, pp = min(parts_ms, key=parts_ms.get)
+        cfg["por_queue"], cfg["por_parts"] = q, pp
+        cfg["xcd_remap"]["porosity"] = r
+        _lib.set_tuning({"por_parts": pp, "por_queue": q, "xcd_remap": {"porosity": r}})
+        if log:
+            log("[autotune] porosity ms by (queue, remap, parts): " + ", ".join(
+                f"{k}={v / steps:.3f}" for k, v in parts_ms.items()))
     # where the DEM interpolation runs changes only the overlap: whole steps
     beside_ms = {}
     if "porosity" in passes:
@@ -100,7 +112,8 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
             log(f"[autotune] step ms with the DEM interpolation beside / after the porosity: "
                 f"{beside_ms[1] / steps:.3f} / {beside_ms[0] / steps:.3f}")
     out = {"tuning": cfg, "ms": table, "por_x_tile_ms": {str(k): v for k, v in xt_ms.items()},
-           "por_parts_ms": {str(k): v for k, v in parts_ms.items()},
+           "por_queue_remap_parts_ms": {f"{q}/{r}/{pp}": v
+                                        for (q, r, pp), v in parts_ms.items()},
            "dem_beside_ms": {str(k): v for k, v in beside_ms.items()},
            "steps_per_candidate": steps}
     if log:
@@ -118,7 +131,8 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
 
 def reset() -> None:
     """Back to the library defaults (256 threads, no remap, x tile 4, the
-    DEM interpolation beside the porosity, 4 porosity waves per work item)."""
+    DEM interpolation beside the porosity, 4 porosity waves per work item
+    taken from one counter)."""
     _lib.set_tuning({"block": {p: 256 for p in _lib.PASS_NAMES},
                      "xcd_remap": {p: 0 for p in _lib.PASS_NAMES}, "por_x_tile": 4,
-                     "dem_beside": 1, "por_parts": 4})
+                     "dem_beside": 1, "por_parts": 4, "por_queue": 1})

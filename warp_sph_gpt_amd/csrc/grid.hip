@@ -46,6 +46,7 @@ sphdem_tuning_t default_tuning() {
     t.por_x_tile = 4;
     t.dem_beside = 1;
     t.por_parts = 4;
+    t.por_queue = 1;
     return t;
 }
 sphdem_tuning_t g_tuning = default_tuning();
@@ -456,6 +457,7 @@ int sphdem_tuning_set(const sphdem_tuning_t *t) {
     SPHDEM_REQUIRE(t->por_x_tile >= 2 && t->por_x_tile <= 6, "por_x_tile must be in 2..6");
     SPHDEM_REQUIRE(t->dem_beside == 0 || t->dem_beside == 1, "dem_beside must be 0 or 1");
     SPHDEM_REQUIRE(t->por_parts >= 1 && t->por_parts <= 16, "por_parts must be in 1..16");
+    SPHDEM_REQUIRE(t->por_queue == 0 || t->por_queue == 1, "por_queue must be 0 or 1");
     g_tuning = *t;
     return SPHDEM_OK;
 }

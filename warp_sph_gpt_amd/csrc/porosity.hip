@@ -802,7 +802,8 @@ __device__ void por_wave(const Prm<T> &p, const SphV<T> &P, const DemV<T> &D, co
 // The chunk list (CHUNKS): cl = {list, L, tag}, count = {chunks, flagged};
 // wave c of the grid takes chunks c, c + waves, ...
 struct ChunkV {
-    const int *list, *L, *tag, *count;
+    const int *list, *L, *tag;
+    int *count;
 };
 
 template <typename T, bool FAST, bool SYM, bool CHUNKS>
@@ -814,7 +815,8 @@ __global__ __launch_bounds__(256, 3) void k_por_stream(Prm<T> p, SphV<T> P, DemV
                                                     GridV<T> gd, const T *Vs, const int *flag,
                                                     T csup, T hu, T thr2, int xt, int remap,
                                                     unsigned long long *col_sum,
-                                                    float col_scale, ChunkV cl, int parts) {
+                                                    float col_scale, ChunkV cl, int parts,
+                                                    int queue) {
     // per wave: 64 staged candidates, 128 for the FP32 matrix-core queue;
     // SYM: a second 128-entry queue for the pairs this wave evaluates for both
     constexpr int kSlot = (FAST && std::is_same<T, float>::value) ? 128 : 64;
@@ -834,10 +836,22 @@ __global__ __launch_bounds__(256, 3) void k_por_stream(Prm<T> p, SphV<T> P, DemV
                       vsubstage + wv * (SYM ? 64 : 1), 0, msubstage + wv * 64};
     const int wave = subject_id(remap) >> 6;
     if constexpr (CHUNKS) {
-        // items (chunk, part), the parts of a chunk on neighbouring waves
-        const int waves = (gridDim.x * blockDim.x) >> 6;
+        // items (chunk, part) in list order, the parts of a chunk next to
+        // each other.  queue 0: wave w takes items w, w + waves, ...;
+        // queue 1: each wave takes the next item from the counter count[2]
+        // (zero at the start of the pass) whenever it finishes one, so the
+        // waves of a workgroup do not wait for each other and the pass ends
+        // within about one item of the last wave's start.
         const int nch = cl.count[0], nfl = cl.count[1];
-        for (int item = wave; item < nch * parts; item += waves) {
+        const int total = nch * parts, waves = (gridDim.x * blockDim.x) >> 6;
+        int item = wave;
+        for (;;) {
+            if (queue != 0) {
+                int got = 0;
+                if (lane == 0) got = atomicAdd(cl.count + 2, 1);
+                item = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
+            }
+            if (item >= total) break;
             const int c = item / parts, part = item - c * parts;
             const int i = cl.list[c] + lane;
             const bool active = i < (c + 1 < nch ? cl.list[c + 1] : nfl);
@@ -845,6 +859,7 @@ __global__ __launch_bounds__(256, 3) void k_por_stream(Prm<T> p, SphV<T> P, DemV
             por_wave<T, FAST, SYM>(p, P, D, gs, gd, Vs, flag, cl.tag, csup, hu, thr2, xt,
                                    col_sum, col_scale, k, active, c, part, parts, lane, wbuf,
                                    vbuf, sq);
+            if (queue == 0) item += waves;
         }
     } else {
         const int k = wave * 64 + lane;
@@ -862,7 +877,8 @@ __global__ __launch_bounds__(256, 3) void k_por_stream(Prm<T> p, SphV<T> P, DemV
 //   RF  first index of L of each row
 //   CS  exclusive scan of the chunk starts (i - RF[row]) % 64 == 0
 //   list[chunk] = its first index of L;  tag[k] = chunk of flagged slot k
-//   (-1 unflagged);  count[0] = chunks, count[1] = flagged slots.
+//   (-1 unflagged);  count[0] = chunks, count[1] = flagged slots,
+//   count[2] = the stream's item counter (zeroed here).
 template <typename T>
 __device__ __forceinline__ bool row_head(const GridV<T> &gs, const int *L, int i) {
     if (i == 0) return true;
@@ -877,6 +893,7 @@ __global__ void k_cl_compact(int n, const int *flag, const int *F, int *L, int *
                              int *count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    if (k == 0) count[2] = 0;   // the stream's item counter
     if (flag[k] != 0) L[F[k]] = k;
     else tag[k] = -1;
     if (k == n - 1) count[1] = F[k] + (flag[k] != 0 ? 1 : 0);
@@ -1002,12 +1019,12 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
             hipLaunchKernelGGL((k_por_stream<T, true, kF, true>), dim3(blocks_for(n, bt)),
                                dim3(bt), 0, st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs),
                                view<T>(gd), (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt,
-                               rm, col, scale, cl, parts);
+                               rm, col, scale, cl, parts, tuning().por_queue);
         else
             hipLaunchKernelGGL((k_por_stream<T, true, kF, false>), dim3(blocks_for(n, bt)),
                                dim3(bt), 0, st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs),
                                view<T>(gd), (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt,
-                               rm, col, scale, cl, 1);
+                               rm, col, scale, cl, 1, 0);
         hipLaunchKernelGGL(k_por_finish<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
                            view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), w->fl_flag,
                            (T *)w->fl_sum, col, std::ldexp(1.0, -e), cs, (T)hu, rm);
@@ -1017,7 +1034,7 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
             hipLaunchKernelGGL(kernel, dim3(blocks_for(n, bt)), dim3(bt), 0, st, prm<T>(p),
                                view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
                                (const T *)w->fl_V, w->fl_flag, cs, (T)hu, thr2, xt, rm,
-                               (unsigned long long *)nullptr, 0.0f, cl, 1);
+                               (unsigned long long *)nullptr, 0.0f, cl, 1, tuning().por_queue);
         };
         if (p->fast_math) {
             if (chunked) launch(k_por_stream<T, true, false, true>);
