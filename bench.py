@@ -241,9 +241,10 @@ def main(argv=None) -> int:
         "graph": graph_run, "surrogate": sur,
         "dominant_pass_counters": counters,
         "passes_ms_per_step": {k: round(v / steps, 4) for k, v in timer.ms.items() if v > 0},
-        # these run on a second stream beside the SPH passes and the porosity:
+        # these run on other streams beside the SPH passes and the porosity:
         # their event intervals include the work they overlap
-        "passes_overlapped": ["dem_csr", "dem_contact", "interaction_dem"],
+        "passes_overlapped": ["dem_csr", "dem_contact", "interaction_dem"] + (
+            ["wall_density"] if _lib.get_tuning()["wall_beside"] else []),
         "diagnostics_fp64": diag,
     }
     print(json.dumps(line), flush=True)

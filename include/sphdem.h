@@ -350,7 +350,11 @@ int sphdem_profile_read(double *ms, int *counts, int n);
  *                    (work list only): 0 wave w takes items w, w + waves,
  *                    ...; 1 every wave takes the next item from one counter
  *                    when it finishes one (default 1).  The same partial
- *                    sums under both values. */
+ *                    sums under both values.
+ *   wall_beside      1: in the fused coupled step, off a Shepard step, the
+ *                    wall density pass runs on a stream of its own beside the
+ *                    fluid density pass (neither reads what the other
+ *                    writes); 0: after it on the caller's stream (default 1) */
 typedef struct {
     int block[SPHDEM_NUM_PASSES];
     int xcd_remap[SPHDEM_NUM_PASSES];
@@ -358,6 +362,7 @@ typedef struct {
     int dem_beside;
     int por_parts;
     int por_queue;
+    int wall_beside;
 } sphdem_tuning_t;
 int sphdem_tuning_get(sphdem_tuning_t *t);
 int sphdem_tuning_set(const sphdem_tuning_t *t);

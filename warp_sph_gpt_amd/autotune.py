@@ -23,7 +23,7 @@ from . import _lib
 BLOCKS = (64, 128, 256)
 REMAPS = (0, 1)
 X_TILES = (3, 4, 5)
-POR_PARTS = (2, 4, 8)
+POR_PARTS = (2, 4, 8, 16)
 POR_QUEUES = (0, 1)
 
 
@@ -100,8 +100,9 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
         if log:
             log("[autotune] porosity ms by (queue, remap, parts): " + ", ".join(
                 f"{k}={v / steps:.3f}" for k, v in parts_ms.items()))
-    # where the DEM interpolation runs changes only the overlap: whole steps
-    beside_ms = {}
+    # where the DEM interpolation and the wall pass run changes only the
+    # overlap: whole steps
+    beside_ms, wall_ms = {}, {}
     if "porosity" in passes:
         for b in (1, 0):
             _lib.set_tuning({"dem_beside": b})
@@ -111,10 +112,20 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
         if log:
             log(f"[autotune] step ms with the DEM interpolation beside / after the porosity: "
                 f"{beside_ms[1] / steps:.3f} / {beside_ms[0] / steps:.3f}")
+    if "wall_density" in passes:
+        for b in (1, 0):
+            _lib.set_tuning({"wall_beside": b})
+            wall_ms[b] = _time_whole_steps(step_fn, steps)
+        cfg["wall_beside"] = min(wall_ms, key=wall_ms.get)
+        _lib.set_tuning({"wall_beside": cfg["wall_beside"]})
+        if log:
+            log(f"[autotune] step ms with the wall pass beside / after the fluid density: "
+                f"{wall_ms[1] / steps:.3f} / {wall_ms[0] / steps:.3f}")
     out = {"tuning": cfg, "ms": table, "por_x_tile_ms": {str(k): v for k, v in xt_ms.items()},
            "por_queue_remap_parts_ms": {f"{q}/{r}/{pp}": v
                                         for (q, r, pp), v in parts_ms.items()},
            "dem_beside_ms": {str(k): v for k, v in beside_ms.items()},
+           "wall_beside_ms": {str(k): v for k, v in wall_ms.items()},
            "steps_per_candidate": steps}
     if log:
         log(f"[autotune] chosen {json.dumps(cfg)}")
@@ -132,7 +143,7 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
 def reset() -> None:
     """Back to the library defaults (256 threads, no remap, x tile 4, the
     DEM interpolation beside the porosity, 4 porosity waves per work item
-    taken from one counter)."""
+    taken from one counter, the wall pass beside the fluid density)."""
     _lib.set_tuning({"block": {p: 256 for p in _lib.PASS_NAMES},
                      "xcd_remap": {p: 0 for p in _lib.PASS_NAMES}, "por_x_tile": 4,
-                     "dem_beside": 1, "por_parts": 4, "por_queue": 1})
+                     "dem_beside": 1, "por_parts": 4, "por_queue": 1, "wall_beside": 1})

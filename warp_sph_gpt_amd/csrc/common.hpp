@@ -45,6 +45,12 @@ struct ProfScope {
 };
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+// A stream for work that may run beside the caller's, with the events that
+// fork it from and join it back to the caller's stream (all null: none).
+struct Beside {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
 inline int blocks_for(long n, int threads) { return (int)((n + threads - 1) / threads); }
 
 // Buckets of a grid over all its instances.

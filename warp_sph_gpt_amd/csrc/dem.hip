@@ -23,7 +23,7 @@ void launch_step_sph(const sphdem_params_t *p, const sphdem_sph_t *P, hipStream_
 template <typename T>
 int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
               const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step, bool coupled,
-              const sphdem_work_t *w, hipStream_t st);
+              const sphdem_work_t *w, hipStream_t st, const Beside &beside);
 template <typename T>
 void launch_pgf_only(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
                      const sphdem_grid_t *gs, const sphdem_grid_t *gb, const sphdem_work_t *w,
@@ -610,27 +610,39 @@ static void launch_step_dem(const sphdem_params_t *p, const sphdem_dem_t *D, hip
 // on the first fused step that is not under graph capture (capture then
 // records the fork and join); until they exist the step runs in one stream.
 // Default priority, non-blocking, events without timing.
+// Per device: the DEM side stream (s, fork, join, marked) and the wall
+// pass's stream (wall.s, wall.fork, wall.join), all at the default priority.
 struct SideStream {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr, marked = nullptr;
+    Beside wall;
 };
 static SideStream *side_stream(hipStream_t caller) {
     static SideStream per_device[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     SideStream &r = per_device[dev];
-    if (r.s && r.fork && r.join && r.marked) return &r;
+    auto ready = [&]() {
+        return r.s && r.fork && r.join && r.marked && r.wall.s && r.wall.fork && r.wall.join;
+    };
+    if (ready()) return &r;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(caller, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
         return nullptr;
-    if (!r.s && hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking) != hipSuccess) r.s = nullptr;
-    if (!r.fork && hipEventCreateWithFlags(&r.fork, hipEventDisableTiming) != hipSuccess)
-        r.fork = nullptr;
-    if (!r.join && hipEventCreateWithFlags(&r.join, hipEventDisableTiming) != hipSuccess)
-        r.join = nullptr;
-    if (!r.marked && hipEventCreateWithFlags(&r.marked, hipEventDisableTiming) != hipSuccess)
-        r.marked = nullptr;
-    return (r.s && r.fork && r.join && r.marked) ? &r : nullptr;
+    auto stream = [](hipStream_t &x) {
+        if (!x && hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) x = nullptr;
+    };
+    auto event = [](hipEvent_t &e) {
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+    };
+    stream(r.s);
+    stream(r.wall.s);
+    for (hipEvent_t *e : {&r.fork, &r.join, &r.marked, &r.wall.fork, &r.wall.join}) event(*e);
+    return ready() ? &r : nullptr;
+}
+// The wall pass's stream when the setting asks for it and the streams exist.
+static Beside wall_beside(SideStream *side) {
+    return (side && tuning().wall_beside) ? side->wall : Beside{};
 }
 
 // The DEM-DEM and DEM-wall contact work of a fused step (counts, scans,
@@ -690,7 +702,7 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
             hipStreamWaitEvent(ds, side->fork, 0);
         }
         rc = dem_contacts<T>(p, D, W, gd, gw, w, ds);
-        if (!rc) rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
+        if (!rc) rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st, wall_beside(side));
         if (rc) {
             join();
             return rc;
@@ -719,7 +731,7 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         return check_launch("coupled_step");
     }
     if (stage == 0) {
-        rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st);
+        rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st, Beside{});
         if (rc) return rc;
         launch_count<T, false>(D, gd, dem_inst(gd), D->radius, p->radius_max, st);
         {
@@ -775,7 +787,7 @@ static int surrogate_prep(const sphdem_params_t *p, const sphdem_sph_t *P, const
                           const sphdem_grid_t *gd, int step, const sphdem_work_t *w,
                           hipStream_t st) {
     const bool coupled = D && D->n > 0;
-    int rc = sph_stage<T>(p, P, B, gs, gb, step, coupled, w, st);
+    int rc = sph_stage<T>(p, P, B, gs, gb, step, coupled, w, st, Beside{});
     if (rc || !coupled) return rc;
     launch_interaction_dem<T>(p, D, gs, gd, w, st);
     launch_fluid_final<T, true, false>(p, P, D, gs, gd, 2.0 * p->h_max, st);

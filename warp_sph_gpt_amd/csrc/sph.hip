@@ -473,16 +473,31 @@ void launch_step_sph(const sphdem_params_t *p, const sphdem_sph_t *P, hipStream_
 
 // SPH part of one step (Simulation.py:17-91), fused: P1 fluid Shepard +
 // density + EOS, P1b wall density + EOS, P2 force (+ -grad p when coupled).
+// `beside` (stream and two events, or nulls): off a Shepard step the wall
+// pass runs on that stream beside the fluid pass -- the fluid density then
+// reads positions and masses only, the wall pass writes only wall fields --
+// and the force pass waits for both.  On a Shepard step the fluid pass reads
+// the old wall densities, so the wall pass follows it on `st`.
 template <typename T>
 int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
               const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step, bool coupled,
-              const sphdem_work_t *w, hipStream_t st) {
+              const sphdem_work_t *w, hipStream_t st, const Beside &beside) {
     const bool shep = step == 0 || (p->shepard_step > 0 && step % p->shepard_step == 0);
+    const bool apart = !shep && beside.s && beside.fork && beside.join;
+    if (apart) {
+        hipEventRecord(beside.fork, st);
+        hipStreamWaitEvent(beside.s, beside.fork, 0);
+        launch_wall_density<T, true>(p, P, B, gs, gb, w, beside.s);
+        hipEventRecord(beside.join, beside.s);
+    }
     if (shep)
         launch_fluid_density<T, true, true, true>(p, P, B, gs, gb, w, st);
     else
         launch_fluid_density<T, false, true, true>(p, P, B, gs, gb, w, st);
-    launch_wall_density<T, true>(p, P, B, gs, gb, w, st);
+    if (apart)
+        hipStreamWaitEvent(st, beside.join, 0);
+    else
+        launch_wall_density<T, true>(p, P, B, gs, gb, w, st);
     if (coupled)
         launch_fluid_force<T, true, true, true>(p, P, B, gs, gb, w, st);
     else
@@ -491,10 +506,10 @@ int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_
 }
 template int sph_stage<float>(const sphdem_params_t *, const sphdem_sph_t *, const sphdem_bnd_t *,
                               const sphdem_grid_t *, const sphdem_grid_t *, int, bool,
-                              const sphdem_work_t *, hipStream_t);
+                              const sphdem_work_t *, hipStream_t, const Beside &);
 template int sph_stage<double>(const sphdem_params_t *, const sphdem_sph_t *, const sphdem_bnd_t *,
                                const sphdem_grid_t *, const sphdem_grid_t *, int, bool,
-                               const sphdem_work_t *, hipStream_t);
+                               const sphdem_work_t *, hipStream_t, const Beside &);
 
 // SPH-scale -grad p alone, from the particle arrays (prep_sphdem entry).
 template <typename T>
@@ -605,7 +620,7 @@ int sphdem_sph_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
     if (rc) return rc;
     SPHDEM_REQUIRE(step >= 0, "step must be nonnegative");
     hipStream_t st = as_stream(stream);
-    DISPATCH(p->precision, rc = sph_stage<T>(p, P, B, gs, gb, step, false, w, st));
+    DISPATCH(p->precision, rc = sph_stage<T>(p, P, B, gs, gb, step, false, w, st, Beside{}));
     if (rc) return rc;
     if (integrate) DISPATCH(p->precision, launch_step_sph<T>(p, P, st));
     return check_launch("sph_step");
