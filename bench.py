@@ -197,17 +197,18 @@ def main(argv=None) -> int:
             pairs = c["pairs_evaluated_per_launch"]
             counters = {"pairs_evaluated_per_launch": pairs,
                         "pair_rate": round(pairs / avg_s, 1),
-                        "valu_issue_frac": c["valu_issue_frac"], "source": src}
-            # the pass is bound by vector-ALU issue, not by HBM bytes: its
-            # FP32 rate against the vector FP32 peak, and the issue fraction
+                        "valu_busy_frac": c["valu_busy_frac"], "source": src}
+            # the pass is bound by the vector ALUs, not by HBM bytes: its
+            # FP32 rate against the vector FP32 peak, and how busy the VALUs
+            # were in the counter run
             tflops = pairs * POROSITY_FLOP_PER_PAIR / avg_s / 1e12
-            roofline["limiter"] = {"what": "valu_issue (model: 4 cycles per wave64 VALU "
-                                           "instruction, 16 per square root)",
-                                   "frac": c["valu_issue_frac"], "source": src}
+            roofline["limiter"] = {"what": "VALU busy (4 SQ_ACTIVE_INST_VALU / (1024 SIMDs x "
+                                           "GRBM_GUI_ACTIVE per XCD))",
+                                   "frac": c["valu_busy_frac"], "source": src}
             roofline["valu"] = {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
                                 "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
                                 "flop_per_pair": POROSITY_FLOP_PER_PAIR,
-                                "pairs_per_launch": pairs, "issue_frac": c["valu_issue_frac"],
+                                "pairs_per_launch": pairs, "busy_frac": c["valu_busy_frac"],
                                 "source": src}
 
     cpu = cpu1 = None
