@@ -386,12 +386,15 @@ int sphdem_reduce_f64(int precision, const void *sph_pos, const void *sph_vel,
  * (K/16)*4*64*4 uint16.  FP32 grids and params only. */
 int sphdem_mpnn_pack(const float *W, int K, void *Bp, void *BTp, void *stream);
 int sphdem_mpnn_to_bf16(const float *x, int n, void *y, void *stream);
-/* Y = resid + act(X W + b), X = [X0 (K0 wide) | X1 (K1 wide)] bf16; Yf
- * (FP32) and/or Yb (bf16) receive Y.  b, resid, X1, Yf, Yb may be NULL. */
+/* Z = X W + b, X = [X0 (K0 wide) | X1 (K1 wide)] bf16.  act nonzero: Yf
+ * (FP32) receives Z, the pre-activation the backward reads, and Yb (bf16)
+ * silu(Z) (resid unused).  act zero: Y = resid + Z goes to Yf and/or Yb.
+ * b, resid, X1, Yf, Yb may be NULL. */
 int sphdem_mpnn_dense_fwd(const void *X0, int K0, const void *X1, int K1, int rows,
-                          const void *Bp, const float *b, int relu, const float *resid,
+                          const void *Bp, const float *b, int act, const float *resid,
                           float *Yf, void *Yb, void *stream);
-/* dX (+)= dZ W^T with dZ = dY * [Ymask > 0] (Ymask NULL: no activation). */
+/* dX (+)= dZ W^T with dZ = dY * silu'(Ymask), Ymask the layer's stored
+ * pre-activation (NULL: no activation, dZ = dY). */
 int sphdem_mpnn_dense_bwd_x(const float *dY, const float *Ymask, int rows, const void *BTp,
                             float *dX0, int K0, float *dX1, int K1, int accumulate,
                             void *stream);
