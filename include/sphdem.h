@@ -50,7 +50,8 @@ typedef struct {
                                     root, as the reference writes them;
                                     1: in the porosity term and the force pass
                                     the hardware square root and reciprocal,
-                                    1/h multiplies (densities stay exact) */
+                                    1/h multiplies, fused multiply-adds
+                                    (densities stay exact) */
 } sphdem_params_t;
 
 /* input/struct.py:3-21 SPHptl */
