@@ -245,7 +245,8 @@ def main(argv=None) -> int:
         # these run on other streams beside the SPH passes and the porosity:
         # their event intervals include the work they overlap
         "passes_overlapped": ["dem_csr", "dem_contact", "interaction_dem"] + (
-            ["wall_density"] if _lib.get_tuning()["wall_beside"] else []),
+            ["wall_density"] if _lib.get_tuning()["wall_beside"] else []) + (
+            ["fluid_force"] if _lib.get_tuning()["force_beside"] else []),
         "diagnostics_fp64": diag,
     }
     print(json.dumps(line), flush=True)

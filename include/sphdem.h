@@ -355,7 +355,13 @@ int sphdem_profile_read(double *ms, int *counts, int n);
  *   wall_beside      1: in the fused coupled step, off a Shepard step, the
  *                    wall density pass runs on a stream of its own beside the
  *                    fluid density pass (neither reads what the other
- *                    writes); 0: after it on the caller's stream (default 1) */
+ *                    writes); 0: after it on the caller's stream (default 1)
+ *   force_beside     1: in the fused coupled step the force pass runs on a
+ *                    stream of its own beside the porosity passes (the
+ *                    force writes accelerations, densities and pressure
+ *                    gradients, the porosity reads positions, m/rho and the
+ *                    DEM state); 0: before them on the caller's stream
+ *                    (default 1) */
 typedef struct {
     int block[SPHDEM_NUM_PASSES];
     int xcd_remap[SPHDEM_NUM_PASSES];
@@ -364,6 +370,7 @@ typedef struct {
     int por_parts;
     int por_queue;
     int wall_beside;
+    int force_beside;
 } sphdem_tuning_t;
 int sphdem_tuning_get(sphdem_tuning_t *t);
 int sphdem_tuning_set(const sphdem_tuning_t *t);

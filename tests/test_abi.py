@@ -84,13 +84,15 @@ def test_tuning_round_trips_and_rejects_out_of_range_values():
     saved = _lib.get_tuning()
     try:
         _lib.set_tuning({"por_parts": 7, "por_x_tile": 3, "dem_beside": 0,
-                         "block": {"porosity": 64}, "por_queue": 0, "wall_beside": 0})
+                         "block": {"porosity": 64}, "por_queue": 0, "wall_beside": 0,
+                         "force_beside": 0})
         t = _lib.get_tuning()
         assert (t["por_parts"], t["por_x_tile"], t["dem_beside"], t["por_queue"],
-                t["wall_beside"]) == (7, 3, 0, 0, 0)
+                t["wall_beside"], t["force_beside"]) == (7, 3, 0, 0, 0, 0)
         assert t["block"]["porosity"] == 64
         for bad in ({"por_parts": 0}, {"por_parts": 17}, {"por_x_tile": 7},
                     {"dem_beside": 2}, {"block": {"porosity": 96}}, {"por_queue": 2}, {"wall_beside": 2},
+                    {"force_beside": 2},
                     {"por_queue": -1}):
             with pytest.raises(ValueError, match="sphdem_tuning_set"):
                 _lib.set_tuning(bad)

@@ -141,3 +141,26 @@ def test_wall_pass_beside_the_fluid_density_gives_the_same_bits(graph):
     for f in ("rho", "pres"):
         np.testing.assert_array_equal(np_(getattr(runs[1].B, f)), np_(getattr(runs[0].B, f)),
                                       err_msg=f"wall {f}")
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_force_pass_beside_the_porosity_gives_the_same_bits(graph):
+    """The force pass on its own stream beside the porosity passes against
+    the force before them on one stream: 25 steps across the Shepard refresh
+    at step 20, eager and replayed, every field bit for bit."""
+    from warp_sph_gpt_amd import _lib
+    solv, st = contact_scene(seed=12)
+    saved = _lib.get_tuning()
+    runs = {}
+    try:
+        for b in (0, 1):
+            _lib.set_tuning({"force_beside": b})
+            eng = CoupledEngine(solv, *(s.to(DEV) for s in st))
+            if graph:
+                eng.capture()
+            eng.run(25)
+            eng.check()
+            runs[b] = eng
+    finally:
+        _lib.set_tuning(saved)
+    assert_same_state(runs[1], runs[0], "force pass beside vs before the porosity")

@@ -100,7 +100,7 @@ class EisphNbr(ctypes.Structure):
 class TuningStruct(ctypes.Structure):
     _fields_ = [("block", c_int * NUM_PASSES), ("xcd_remap", c_int * NUM_PASSES),
                 ("por_x_tile", c_int), ("dem_beside", c_int), ("por_parts", c_int),
-                ("por_queue", c_int), ("wall_beside", c_int)]
+                ("por_queue", c_int), ("wall_beside", c_int), ("force_beside", c_int)]
 
 
 _SIGNATURES = {
@@ -419,7 +419,7 @@ def get_tuning() -> dict:
             "xcd_remap": {n: int(t.xcd_remap[i]) for i, n in enumerate(PASS_NAMES)},
             "por_x_tile": int(t.por_x_tile), "dem_beside": int(t.dem_beside),
             "por_parts": int(t.por_parts), "por_queue": int(t.por_queue),
-            "wall_beside": int(t.wall_beside)}
+            "wall_beside": int(t.wall_beside), "force_beside": int(t.force_beside)}
 
 
 def set_tuning(cfg: dict) -> None:
@@ -434,6 +434,7 @@ def set_tuning(cfg: dict) -> None:
     t.por_parts = int(cfg.get("por_parts", cur["por_parts"]))
     t.por_queue = int(cfg.get("por_queue", cur["por_queue"]))
     t.wall_beside = int(cfg.get("wall_beside", cur["wall_beside"]))
+    t.force_beside = int(cfg.get("force_beside", cur["force_beside"]))
     call("sphdem_tuning_set", t)
 
 

@@ -27,18 +27,25 @@ POR_PARTS = (2, 4, 8, 16)
 POR_QUEUES = (0, 1)
 
 
-def _time_whole_steps(step_fn, steps: int) -> float:
-    """Milliseconds of `steps` steps on the current stream (every pass, both
-    streams: the side stream joins back before a step ends)."""
-    step_fn()
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(steps):
+def _median_step_ms(step_fn, steps: int, knob: str) -> dict:
+    """Median milliseconds of one whole step (every pass, all streams: the
+    side streams join back before a step ends) with `knob` at 1 and at 0.
+    Each setting runs four times, alternating (1 0 0 1 1 0 0 1), for one
+    settling step and steps + 2 timed ones; the median over a setting's
+    timed steps leaves out the odd costlier step (a Shepard refresh)."""
+    m = steps + 2
+    times = {1: [], 0: []}
+    for b in (1, 0, 0, 1, 1, 0, 0, 1):
+        _lib.set_tuning({knob: b})
         step_fn()
-    b.record()
-    torch.cuda.synchronize()
-    return a.elapsed_time(b)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(m + 1)]
+        ev[0].record()
+        for i in range(m):
+            step_fn()
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        times[b] += [ev[i].elapsed_time(ev[i + 1]) for i in range(m)]
+    return {b: sorted(t)[len(t) // 2] for b, t in times.items()}
 
 
 def _time_steps(step_fn, steps: int) -> dict:
@@ -62,6 +69,10 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
         if key in stored:
             _lib.set_tuning(stored[key]["tuning"])
             return stored[key]
+    # the per-pass timings run the force pass alone (beside the porosity the
+    # two share the chip); where it runs is chosen on whole steps at the end
+    force_beside = _lib.get_tuning()["force_beside"]
+    _lib.set_tuning({"force_beside": 0})
     table = {p: {} for p in passes}
     for b in BLOCKS:
         for r in REMAPS:
@@ -100,32 +111,29 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
         if log:
             log("[autotune] porosity ms by (queue, remap, parts): " + ", ".join(
                 f"{k}={v / steps:.3f}" for k, v in parts_ms.items()))
-    # where the DEM interpolation and the wall pass run changes only the
-    # overlap: whole steps
-    beside_ms, wall_ms = {}, {}
-    if "porosity" in passes:
-        for b in (1, 0):
-            _lib.set_tuning({"dem_beside": b})
-            beside_ms[b] = _time_whole_steps(step_fn, steps)
-        cfg["dem_beside"] = min(beside_ms, key=beside_ms.get)
-        _lib.set_tuning({"dem_beside": cfg["dem_beside"]})
+    # where the DEM interpolation, the wall pass and the force pass run
+    # changes only the overlap: whole steps, each switch on its own
+    whole = {}
+    for knob, needs, what in (
+            ("dem_beside", ("porosity",), "the DEM interpolation beside / after the porosity"),
+            ("wall_beside", ("wall_density",), "the wall pass beside / after the fluid density"),
+            ("force_beside", ("porosity", "fluid_force"),
+             "the force pass beside / before the porosity")):
+        if not all(p in passes for p in needs):
+            continue
+        whole[knob] = _median_step_ms(step_fn, steps, knob)
+        cfg[knob] = min(whole[knob], key=whole[knob].get)
+        _lib.set_tuning({knob: cfg[knob]})
         if log:
-            log(f"[autotune] step ms with the DEM interpolation beside / after the porosity: "
-                f"{beside_ms[1] / steps:.3f} / {beside_ms[0] / steps:.3f}")
-    if "wall_density" in passes:
-        for b in (1, 0):
-            _lib.set_tuning({"wall_beside": b})
-            wall_ms[b] = _time_whole_steps(step_fn, steps)
-        cfg["wall_beside"] = min(wall_ms, key=wall_ms.get)
-        _lib.set_tuning({"wall_beside": cfg["wall_beside"]})
-        if log:
-            log(f"[autotune] step ms with the wall pass beside / after the fluid density: "
-                f"{wall_ms[1] / steps:.3f} / {wall_ms[0] / steps:.3f}")
+            log(f"[autotune] median step ms with {what}: "
+                f"{whole[knob][1]:.3f} / {whole[knob][0]:.3f}")
+    if "force_beside" not in whole:
+        _lib.set_tuning({"force_beside": force_beside})
     out = {"tuning": cfg, "ms": table, "por_x_tile_ms": {str(k): v for k, v in xt_ms.items()},
            "por_queue_remap_parts_ms": {f"{q}/{r}/{pp}": v
                                         for (q, r, pp), v in parts_ms.items()},
-           "dem_beside_ms": {str(k): v for k, v in beside_ms.items()},
-           "wall_beside_ms": {str(k): v for k, v in wall_ms.items()},
+           **{f"{k}_median_step_ms": {str(b): v for b, v in ms.items()}
+              for k, ms in whole.items()},
            "steps_per_candidate": steps}
     if log:
         log(f"[autotune] chosen {json.dumps(cfg)}")
@@ -143,7 +151,9 @@ def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | N
 def reset() -> None:
     """Back to the library defaults (256 threads, no remap, x tile 4, the
     DEM interpolation beside the porosity, 4 porosity waves per work item
-    taken from one counter, the wall pass beside the fluid density)."""
+    taken from one counter, the wall pass beside the fluid density, the
+    force pass beside the porosity)."""
     _lib.set_tuning({"block": {p: 256 for p in _lib.PASS_NAMES},
                      "xcd_remap": {p: 0 for p in _lib.PASS_NAMES}, "por_x_tile": 4,
-                     "dem_beside": 1, "por_parts": 4, "por_queue": 1, "wall_beside": 1})
+                     "dem_beside": 1, "por_parts": 4, "por_queue": 1, "wall_beside": 1,
+                     "force_beside": 1})

@@ -23,7 +23,8 @@ void launch_step_sph(const sphdem_params_t *p, const sphdem_sph_t *P, hipStream_
 template <typename T>
 int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
               const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step, bool coupled,
-              const sphdem_work_t *w, hipStream_t st, const Beside &beside);
+              const sphdem_work_t *w, hipStream_t st, const Beside &beside,
+              const Beside &force);
 template <typename T>
 void launch_pgf_only(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
                      const sphdem_grid_t *gs, const sphdem_grid_t *gb, const sphdem_work_t *w,
@@ -615,7 +616,7 @@ static void launch_step_dem(const sphdem_params_t *p, const sphdem_dem_t *D, hip
 struct SideStream {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr, marked = nullptr;
-    Beside wall;
+    Beside wall, force;   // the force pass runs on the wall pass's stream
 };
 static SideStream *side_stream(hipStream_t caller) {
     static SideStream per_device[64];
@@ -623,7 +624,8 @@ static SideStream *side_stream(hipStream_t caller) {
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     SideStream &r = per_device[dev];
     auto ready = [&]() {
-        return r.s && r.fork && r.join && r.marked && r.wall.s && r.wall.fork && r.wall.join;
+        return r.s && r.fork && r.join && r.marked && r.wall.s && r.wall.fork && r.wall.join &&
+               r.force.s && r.force.fork && r.force.join;
     };
     if (ready()) return &r;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -637,12 +639,19 @@ static SideStream *side_stream(hipStream_t caller) {
     };
     stream(r.s);
     stream(r.wall.s);
-    for (hipEvent_t *e : {&r.fork, &r.join, &r.marked, &r.wall.fork, &r.wall.join}) event(*e);
+    r.force.s = r.wall.s;
+    for (hipEvent_t *e : {&r.fork, &r.join, &r.marked, &r.wall.fork, &r.wall.join, &r.force.fork,
+                          &r.force.join})
+        event(*e);
     return ready() ? &r : nullptr;
 }
 // The wall pass's stream when the setting asks for it and the streams exist.
 static Beside wall_beside(SideStream *side) {
     return (side && tuning().wall_beside) ? side->wall : Beside{};
+}
+// The force pass's stream, likewise.
+static Beside force_beside(SideStream *side) {
+    return (side && tuning().force_beside) ? side->force : Beside{};
 }
 
 // The DEM-DEM and DEM-wall contact work of a fused step (counts, scans,
@@ -701,9 +710,18 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
             hipEventRecord(side->fork, st);
             hipStreamWaitEvent(ds, side->fork, 0);
         }
+        // beside the porosity (which reads positions, m/rho from the
+        // density pass and DEM state) the force pass writes accelerations,
+        // densities and pressure gradients; the DEM interpolation and the
+        // final update wait for it
+        const Beside fb = force_beside(side);
+        auto force_joined = [&](hipStream_t s) {
+            if (fb.s) hipStreamWaitEvent(s, fb.join, 0);
+        };
         rc = dem_contacts<T>(p, D, W, gd, gw, w, ds);
-        if (!rc) rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st, wall_beside(side));
+        if (!rc) rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st, wall_beside(side), fb);
         if (rc) {
+            force_joined(st);
             join();
             return rc;
         }
@@ -715,15 +733,18 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         rc = porosity<T>(p, P, D, gs, gd, w, csup, p->h_uniform, st, near,
                          beside ? side->marked : nullptr);
         if (rc) {
+            force_joined(st);
             join();
             return rc;
         }
         if (beside) {
             hipStreamWaitEvent(ds, side->marked, 0);
+            force_joined(ds);
             launch_interaction_dem<T>(p, D, gs, gd, w, ds, near);
             join();
         } else {
             join();
+            force_joined(st);
             launch_interaction_dem<T>(p, D, gs, gd, w, st, near);
         }
         launch_fluid_final<T, true, true>(p, P, D, gs, gd, csup, st, near ? w->fl_flag : nullptr);
@@ -731,7 +752,7 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         return check_launch("coupled_step");
     }
     if (stage == 0) {
-        rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st, Beside{});
+        rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st, Beside{}, Beside{});
         if (rc) return rc;
         launch_count<T, false>(D, gd, dem_inst(gd), D->radius, p->radius_max, st);
         {
@@ -787,7 +808,7 @@ static int surrogate_prep(const sphdem_params_t *p, const sphdem_sph_t *P, const
                           const sphdem_grid_t *gd, int step, const sphdem_work_t *w,
                           hipStream_t st) {
     const bool coupled = D && D->n > 0;
-    int rc = sph_stage<T>(p, P, B, gs, gb, step, coupled, w, st, Beside{});
+    int rc = sph_stage<T>(p, P, B, gs, gb, step, coupled, w, st, Beside{}, Beside{});
     if (rc || !coupled) return rc;
     launch_interaction_dem<T>(p, D, gs, gd, w, st);
     launch_fluid_final<T, true, false>(p, P, D, gs, gd, 2.0 * p->h_max, st);

@@ -48,6 +48,7 @@ sphdem_tuning_t default_tuning() {
     t.por_parts = 4;
     t.por_queue = 1;
     t.wall_beside = 1;
+    t.force_beside = 1;
     return t;
 }
 sphdem_tuning_t g_tuning = default_tuning();
@@ -460,6 +461,7 @@ int sphdem_tuning_set(const sphdem_tuning_t *t) {
     SPHDEM_REQUIRE(t->por_parts >= 1 && t->por_parts <= 16, "por_parts must be in 1..16");
     SPHDEM_REQUIRE(t->por_queue == 0 || t->por_queue == 1, "por_queue must be 0 or 1");
     SPHDEM_REQUIRE(t->wall_beside == 0 || t->wall_beside == 1, "wall_beside must be 0 or 1");
+    SPHDEM_REQUIRE(t->force_beside == 0 || t->force_beside == 1, "force_beside must be 0 or 1");
     g_tuning = *t;
     return SPHDEM_OK;
 }

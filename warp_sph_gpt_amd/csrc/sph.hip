@@ -40,8 +40,8 @@ template <typename T> struct FastDW {
 // Fluid subjects: Shepard filter and/or density (KERNEL_rho.py:6-82).
 //  SHEP: flt_i = sum (m_j/rho_j) W over fluid and wall, stored in P.flt.
 //  DENS: rho_i = sum m_j W / flt_i.
-//  WORK: the fused step -- new density, Tait pressure and velocity go to the
-//        sorted working arrays (P.rho is still read as the old density by
+//  WORK: the fused step -- new density, Tait pressure, velocity and volume
+//        m/rho go to the sorted working arrays (P.rho is still read as the old density by
 //        the Shepard sum of other threads, so it is published by the force
 //        pass); P.pres is written here.  Without WORK density goes to P.rho.
 //  The density keeps the reference's arithmetic in both math modes: the
@@ -91,6 +91,7 @@ __global__ __launch_bounds__(kThreads) void k_fluid_density(Prm<T> p, SphV<T> P,
             w.fl_pres[k] = pr;
             P.pres[i] = pr;
             w.fl_vel[k] = V4<T>{P.vel[3 * i], P.vel[3 * i + 1], P.vel[3 * i + 2], r};
+            w.fl_V[k] = me.w / r;
         } else {
             P.rho[i] = r;
         }
@@ -256,7 +257,6 @@ __global__ __launch_bounds__(kThreads) void k_fluid_force(Prm<T> p, SphV<T> P, B
     if (PGF) {
         st3(P.pgf, i, pgf);
         w.fl_pgf[k] = V4<T>{pgf.x, pgf.y, pgf.z, rhoi};
-        w.fl_V[k] = me.w / rhoi;
     }
 }
 
@@ -477,11 +477,15 @@ void launch_step_sph(const sphdem_params_t *p, const sphdem_sph_t *P, hipStream_
 // pass runs on that stream beside the fluid pass -- the fluid density then
 // reads positions and masses only, the wall pass writes only wall fields --
 // and the force pass waits for both.  On a Shepard step the fluid pass reads
-// the old wall densities, so the wall pass follows it on `st`.
+// the old wall densities, so the wall pass follows it on `st`.  `force`
+// (the same, or nulls): the force pass runs on its stream after both density
+// passes, and the caller makes every reader of the force's outputs wait for
+// `force.join` (recorded whenever `force.s` is set, also on an error).
 template <typename T>
 int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_t *B,
               const sphdem_grid_t *gs, const sphdem_grid_t *gb, int step, bool coupled,
-              const sphdem_work_t *w, hipStream_t st, const Beside &beside) {
+              const sphdem_work_t *w, hipStream_t st, const Beside &beside,
+              const Beside &force) {
     const bool shep = step == 0 || (p->shepard_step > 0 && step % p->shepard_step == 0);
     const bool apart = !shep && beside.s && beside.fork && beside.join;
     if (apart) {
@@ -498,18 +502,27 @@ int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_
         hipStreamWaitEvent(st, beside.join, 0);
     else
         launch_wall_density<T, true>(p, P, B, gs, gb, w, st);
+    hipStream_t fs = st;
+    if (force.s) {
+        hipEventRecord(force.fork, st);
+        hipStreamWaitEvent(force.s, force.fork, 0);
+        fs = force.s;
+    }
     if (coupled)
-        launch_fluid_force<T, true, true, true>(p, P, B, gs, gb, w, st);
+        launch_fluid_force<T, true, true, true>(p, P, B, gs, gb, w, fs);
     else
-        launch_fluid_force<T, true, false, true>(p, P, B, gs, gb, w, st);
+        launch_fluid_force<T, true, false, true>(p, P, B, gs, gb, w, fs);
+    if (force.s) hipEventRecord(force.join, force.s);
     return check_launch("sph_stage");
 }
 template int sph_stage<float>(const sphdem_params_t *, const sphdem_sph_t *, const sphdem_bnd_t *,
                               const sphdem_grid_t *, const sphdem_grid_t *, int, bool,
-                              const sphdem_work_t *, hipStream_t, const Beside &);
+                              const sphdem_work_t *, hipStream_t, const Beside &,
+                              const Beside &);
 template int sph_stage<double>(const sphdem_params_t *, const sphdem_sph_t *, const sphdem_bnd_t *,
                                const sphdem_grid_t *, const sphdem_grid_t *, int, bool,
-                               const sphdem_work_t *, hipStream_t, const Beside &);
+                               const sphdem_work_t *, hipStream_t, const Beside &,
+                              const Beside &);
 
 // SPH-scale -grad p alone, from the particle arrays (prep_sphdem entry).
 template <typename T>
@@ -620,7 +633,7 @@ int sphdem_sph_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
     if (rc) return rc;
     SPHDEM_REQUIRE(step >= 0, "step must be nonnegative");
     hipStream_t st = as_stream(stream);
-    DISPATCH(p->precision, rc = sph_stage<T>(p, P, B, gs, gb, step, false, w, st, Beside{}));
+    DISPATCH(p->precision, rc = sph_stage<T>(p, P, B, gs, gb, step, false, w, st, Beside{}, Beside{}));
     if (rc) return rc;
     if (integrate) DISPATCH(p->precision, launch_step_sph<T>(p, P, st));
     return check_launch("sph_step");
