@@ -276,9 +276,10 @@ int sphdem_sph_step(const sphdem_params_t *p, sphdem_sph_t *P, sphdem_bnd_t *B,
                     int integrate, const sphdem_work_t *w, void *stream);
 
 /* Per-device set-up (no reference counterpart; Warp creates its streams on
- * wp.init()): the side streams and events the fused step's DEM contact work
- * and its wall density pass run on (default priority), and the device
- * address of sphdem_exclusive_scan_noalloc's array, for the current device.  Call once per device
+ * wp.init()): the side streams and events the fused step's DEM contact work,
+ * its wall density pass and its force pass run on (default priority), and the
+ * device address of sphdem_exclusive_scan_noalloc's array, for the current
+ * device.  Call once per device
  * before capturing a coupled step into a hipGraph, outside capture; a first
  * step outside capture does the same.  Until then a captured step records a
  * single-stream step (the same results, without the overlap). */
