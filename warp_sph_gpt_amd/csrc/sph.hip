@@ -489,30 +489,30 @@ int sph_stage(const sphdem_params_t *p, const sphdem_sph_t *P, const sphdem_bnd_
     const bool shep = step == 0 || (p->shepard_step > 0 && step % p->shepard_step == 0);
     const bool apart = !shep && beside.s && beside.fork && beside.join;
     if (apart) {
-        hipEventRecord(beside.fork, st);
-        hipStreamWaitEvent(beside.s, beside.fork, 0);
+        (void)hipEventRecord(beside.fork, st);
+        (void)hipStreamWaitEvent(beside.s, beside.fork, 0);
         launch_wall_density<T, true>(p, P, B, gs, gb, w, beside.s);
-        hipEventRecord(beside.join, beside.s);
+        (void)hipEventRecord(beside.join, beside.s);
     }
     if (shep)
         launch_fluid_density<T, true, true, true>(p, P, B, gs, gb, w, st);
     else
         launch_fluid_density<T, false, true, true>(p, P, B, gs, gb, w, st);
     if (apart)
-        hipStreamWaitEvent(st, beside.join, 0);
+        (void)hipStreamWaitEvent(st, beside.join, 0);
     else
         launch_wall_density<T, true>(p, P, B, gs, gb, w, st);
     hipStream_t fs = st;
     if (force.s) {
-        hipEventRecord(force.fork, st);
-        hipStreamWaitEvent(force.s, force.fork, 0);
+        (void)hipEventRecord(force.fork, st);
+        (void)hipStreamWaitEvent(force.s, force.fork, 0);
         fs = force.s;
     }
     if (coupled)
         launch_fluid_force<T, true, true, true>(p, P, B, gs, gb, w, fs);
     else
         launch_fluid_force<T, true, false, true>(p, P, B, gs, gb, w, fs);
-    if (force.s) hipEventRecord(force.join, force.s);
+    if (force.s) (void)hipEventRecord(force.join, force.s);
     return check_launch("sph_stage");
 }
 template int sph_stage<float>(const sphdem_params_t *, const sphdem_sph_t *, const sphdem_bnd_t *,
