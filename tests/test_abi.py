@@ -27,7 +27,7 @@ def struct_fields(name):
     out = []
     for decl in body.split(";"):
         words = [w for w in re.split(r"[\s,*]+", decl) if w]
-        out.extend(w for w in words if w not in ("int", "double", "void", "const"))
+        out.extend(w.split("[")[0] for w in words if w not in ("int", "double", "void", "const"))
     return out
 
 
@@ -47,7 +47,8 @@ def test_library_exports_every_declared_function():
 @pytest.mark.parametrize("cname,pyname", [
     ("sphdem_params_t", "Params"), ("sphdem_sph_t", "SphView"), ("sphdem_bnd_t", "BndView"),
     ("sphdem_dem_t", "DemView"), ("sphdem_dembnd_t", "DemBndView"),
-    ("sphdem_grid_t", "GridStruct"), ("sphdem_work_t", "WorkStruct")])
+    ("sphdem_grid_t", "GridStruct"), ("sphdem_work_t", "WorkStruct"),
+    ("sphdem_tuning_t", "TuningStruct")])
 def test_ctypes_structures_follow_the_header(cname, pyname):
     fields = struct_fields(cname)
     got = [f[0] for f in getattr(_lib, pyname)._fields_]
