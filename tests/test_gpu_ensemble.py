@@ -90,7 +90,8 @@ def test_launch_shapes_change_no_bit():
     runs = []
     for cfg in ({}, {"block": {p: 64 for p in _lib.TUNABLE_PASSES},
                      "xcd_remap": {p: 1 for p in _lib.TUNABLE_PASSES}, "por_x_tile": 2},
-                {"block": {p: 128 for p in _lib.TUNABLE_PASSES}, "por_x_tile": 6}):
+                {"block": {p: 128 for p in _lib.TUNABLE_PASSES}, "por_x_tile": 6},
+                {"force_beside": 0, "wall_beside": 0, "dem_beside": 0}):
         autotune.reset()
         _lib.set_tuning(cfg)
         ens = EnsembleEngine(solv, sc, device=DEV)
