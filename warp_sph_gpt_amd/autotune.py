@@ -7,7 +7,10 @@ on the live simulation.  For each candidate it enqueues a few real steps
 with per-pass HIP-event timing and keeps, pass by pass, the fastest shape.
 The porosity's waves per work item (fast FP32 path) regroup its fixed-point
 partial sums, within that path's tolerance; they are timed the same way,
-together with the order in which the waves take the work items.
+together with the order in which the waves take the work items.  Where the
+fused step runs its DEM interpolation, wall pass and force pass (beside
+other passes on library streams, or in sequence) changes only the overlap;
+each is chosen on the median of single-step times of the two settings.
 
     shapes = autotune(engine.step, steps=3)
 """
@@ -60,9 +63,12 @@ def _time_steps(step_fn, steps: int) -> dict:
 
 def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | None = None,
              key: str | None = None, log=None) -> dict:
-    """Choose block size and tile order per pass, then the porosity x tile.
-    Leaves the library set to the result and returns it (with the timings).
-    With `cache` and `key`, a stored result for `key` is applied instead."""
+    """Choose block size and tile order per pass, then the porosity's x tile,
+    its waves per work item and how the waves take the items, then where the
+    DEM interpolation, the wall pass and the force pass run (each on whole
+    steps).  Leaves the library set to the result and returns it (with the
+    timings).  With `cache` and `key`, a stored result for `key` is applied
+    instead."""
     if cache and key and os.path.exists(cache):
         with open(cache) as f:
             stored = json.load(f)
