@@ -702,13 +702,13 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         hipStream_t ds = side ? side->s : st;
         auto join = [&]() {
             if (side) {
-                hipEventRecord(side->join, ds);
-                hipStreamWaitEvent(st, side->join, 0);
+                (void)hipEventRecord(side->join, ds);
+                (void)hipStreamWaitEvent(st, side->join, 0);
             }
         };
         if (side) {
-            hipEventRecord(side->fork, st);
-            hipStreamWaitEvent(ds, side->fork, 0);
+            (void)hipEventRecord(side->fork, st);
+            (void)hipStreamWaitEvent(ds, side->fork, 0);
         }
         // beside the porosity (which reads positions, m/rho from the
         // density pass and DEM state) the force pass writes accelerations,
@@ -716,7 +716,7 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
         // final update wait for it
         const Beside fb = force_beside(side);
         auto force_joined = [&](hipStream_t s) {
-            if (fb.s) hipStreamWaitEvent(s, fb.join, 0);
+            if (fb.s) (void)hipStreamWaitEvent(s, fb.join, 0);
         };
         rc = dem_contacts<T>(p, D, W, gd, gw, w, ds);
         if (!rc) rc = sph_stage<T>(p, P, B, gs, gb, step, true, w, st, wall_beside(side), fb);
@@ -738,7 +738,7 @@ static int coupled_step(const sphdem_params_t *p, const sphdem_sph_t *P, const s
             return rc;
         }
         if (beside) {
-            hipStreamWaitEvent(ds, side->marked, 0);
+            (void)hipStreamWaitEvent(ds, side->marked, 0);
             force_joined(ds);
             launch_interaction_dem<T>(p, D, gs, gd, w, ds, near);
             join();

@@ -984,7 +984,7 @@ int launch_porosity_tiled(const sphdem_params_t *p, const sphdem_sph_t *P, const
                        view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), cs, w->fl_flag, dem_near,
                        rm);
     // the flags are final here: work that needs only them may start
-    if (marked) hipEventRecord(marked, st);
+    if (marked) (void)hipEventRecord(marked, st);
     const T thr2 = sqrt_threshold<T>(cs);
     // the chunk list of the flagged slots (fl_chunk: F, L, R, RF, CS, list,
     // tag of n ints each; the counts in fl_count[0..1])
