@@ -55,12 +55,14 @@ def test_ctypes_structures_follow_the_header(cname, pyname):
     assert got == fields, (cname, fields, got)
 
 
+# the repository's integration notes, whose first python block is the
+# reference-side ctypes binding a maintainer would add
 INTEGRATION = (ROOT / "INTEGRATION.md").read_text()
 
 
 def integration_struct(name):
-    """Run the ctypes class `name` of INTEGRATION.md's reference-side binding
-    (its definition only) and return it."""
+    """Run the ctypes class `name` of the integration notes' reference-side
+    binding (its definition only) and return it."""
     code = re.search(r"```python\n(.*?)```", INTEGRATION, re.S).group(1)
     start = code.index(f"class {name}(ctypes.Structure)")
     end = code.index("\n\n", start)
