@@ -70,7 +70,7 @@ def main(argv=None) -> int:
     ap.add_argument("--graph", action="store_true",
                     help="replay captured hipGraphs (always on for the c3 ensemble)")
     ap.add_argument("--instances", type=int, default=64, help="c3: number of instances")
-    ap.add_argument("--tune-steps", type=int, default=2,
+    ap.add_argument("--tune-steps", type=int, default=3,
                     help="steps per autotuner candidate during warm-up (0: library defaults)")
     ap.add_argument("--surrogate-sim-time", type=float, default=0.5,
                     help="simulated seconds of the C1 surrogate-vs-plain rollout (0: skip)")

@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import json
 import os
+import statistics
 
 import torch
 
@@ -52,13 +53,19 @@ def _median_step_ms(step_fn, steps: int, knob: str) -> dict:
 
 
 def _time_steps(step_fn, steps: int) -> dict:
+    """Per-pass milliseconds of `steps` steps: `steps` times the median over
+    the steps of each pass's time, so that the odd costlier step (a Shepard
+    refresh) does not decide a comparison (with 3 or more steps)."""
     step_fn()                      # settle the new shapes (first-launch costs)
     torch.cuda.synchronize()
-    with _lib.PassTimer() as t:
-        for _ in range(steps):
+    per = []
+    for _ in range(steps):
+        with _lib.PassTimer() as t:
             step_fn()
-        torch.cuda.synchronize()
-    return dict(t.ms)
+            torch.cuda.synchronize()
+        per.append(t.ms)
+    return {k: steps * statistics.median(p.get(k, 0.0) for p in per)
+            for k in set().union(*per)}
 
 
 def autotune(step_fn, steps: int = 3, passes=_lib.TUNABLE_PASSES, cache: str | None = None,
