@@ -99,7 +99,7 @@ typedef struct {
     int n;                /* points in the last build */
     int *point_cells;     /* [n]   bucket key of original point i */
     int *point_ids;       /* [n]   sorted slot -> original index (stable) */
-    int *rank;            /* [n]   scratch */
+    int *rank;            /* [n]   after a build: the sorted slot of original point i */
     int *cell_lo;         /* [ncell] first sorted slot of each bucket */
     int *cell_hi;         /* [ncell] one past the last slot of each bucket */
     void *sorted;         /* [n*4] (x, y, z, w) in sorted order, w from `w` */

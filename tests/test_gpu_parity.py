@@ -75,6 +75,9 @@ def test_grid_keys_order_bounds_bit_exact(case):
     np.testing.assert_array_equal(np_(s), og.cell_starts)
     np.testing.assert_array_equal(np_(e), og.cell_ends)
     np.testing.assert_array_equal(np_(gg.sorted_points)[:, :3], pts[og.point_ids])
+    # the build leaves each point's sorted slot in rank (the inverse order)
+    n = len(pts)
+    np.testing.assert_array_equal(np_(gg._rank)[:n][og.point_ids], np.arange(n))
 
 
 def test_neighbour_counts_bit_exact():

@@ -377,36 +377,51 @@ __global__ __launch_bounds__(kDemThreads) void k_interaction_dem(Prm<T> p, DemV<
     }
 }
 
-// kernel/KERNEL_SPHDEM_interaction.py:190-226 (INTER) fused with
-// kernel/KERNEL_step.py:6-22 (STEP): both only touch the subject's own fields.
-template <typename T, bool INTER, bool STEP>
+// kernel/KERNEL_SPHDEM_interaction.py:190-226 (INTER) and
+// kernel/KERNEL_step.py:6-22 (STEP): both only touch the subject's own
+// fields.  Two launches: the drag reaction of the near-DEM subjects in
+// sorted order, whose lanes are neighbours and walk the same DEM buckets,
+// leaving acc_dem; then the update in original particle order, whose reads
+// and writes of the particle arrays are contiguous.
 // near (or null): the porosity mark's near-DEM flag of each sorted fluid
 // slot, from this step's positions.  The mark ran this very DEM query and
-// tested the same distance, so an unflagged subject has no term here and
-// skips the query (fd stays +0, as its loop would leave it).
-__global__ __launch_bounds__(kThreads) void k_fluid_final(Prm<T> p, SphV<T> P, DemV<T> D,
-                                                          GridV<T> gs, GridV<T> gd, T csup,
-                                                          const int *near, int remap) {
+// tested the same distance, so an unflagged subject has no term here: its
+// fd is +0 and so is fd / (rho eps), which the update uses without a query.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fluid_drag(Prm<T> p, SphV<T> P, DemV<T> D,
+                                                         GridV<T> gs, GridV<T> gd, T csup,
+                                                         const int *near, int remap) {
     const int k = subject_id(remap);
-    if (k >= gs.n) return;
+    if (k >= gs.n || (near && !near[k])) return;
     const int i = gs.ids[k];
     gd = at_inst(gd, inst_of(gs, k));
+    const V4<T> me = gs.s[k];
+    V3<T> fd = mk3<T>(0, 0, 0);
+    query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
+        for (int s = lo; s < hi; ++s) {
+            const int b = gd.ids[s];
+            const V4<T> q = gd.s[s];
+            const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
+            const T hb = D.h[b];
+            if (dist < (T)2.0 * hb && D.flt_s[b] > (T)1.0e-12)
+                fd = fd - (ld3(D.drag, b) * wendland_w(dist, hb)) / D.flt_s[b];
+        }
+    });
+    st3(P.acc_dem, i, fd / (P.rho[i] * P.porosity[i]));
+}
+// slot[i]: original point i's sorted slot (the grid build's inverse order).
+template <typename T, bool INTER, bool STEP>
+__global__ __launch_bounds__(kThreads) void k_fluid_final(Prm<T> p, SphV<T> P, const int *slot,
+                                                          int n, const int *near, int remap) {
+    const int i = subject_id(remap);
+    if (i >= n) return;
     V3<T> acc = ld3(P.acc, i);
     if (INTER) {
-        const V4<T> me = gs.s[k];
-        V3<T> fd = mk3<T>(0, 0, 0);
-        if (!near || near[k]) query_runs(gd, me.x, me.y, me.z, csup, [&](int lo, int hi) {
-            for (int s = lo; s < hi; ++s) {
-                const int b = gd.ids[s];
-                const V4<T> q = gd.s[s];
-                const T dist = length(mk3(me.x - q.x, me.y - q.y, me.z - q.z));
-                const T hb = D.h[b];
-                if (dist < (T)2.0 * hb && D.flt_s[b] > (T)1.0e-12)
-                    fd = fd - (ld3(D.drag, b) * wendland_w(dist, hb)) / D.flt_s[b];
-            }
-        });
-        const V3<T> ad = fd / (P.rho[i] * P.porosity[i]);
-        st3(P.acc_dem, i, ad);
+        V3<T> ad = mk3<T>(0, 0, 0);
+        if (!near || near[slot[i]])
+            ad = ld3(P.acc_dem, i);
+        else
+            st3(P.acc_dem, i, ad);
         acc = acc + ad;
         st3(P.acc, i, acc);
     }
@@ -593,9 +608,13 @@ static void launch_fluid_final(const sphdem_params_t *p, const sphdem_sph_t *P,
     if (gs->n == 0) return;
     ProfScope prof(SPHDEM_PASS_FLUID_FINAL, st);
     const int bt = tuned_block(SPHDEM_PASS_FLUID_FINAL);
+    const int rm = tuned_remap(SPHDEM_PASS_FLUID_FINAL);
+    if (INTER)
+        hipLaunchKernelGGL(k_fluid_drag<T>, dim3(blocks_for(gs->n, bt)), dim3(bt), 0, st,
+                           prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd), (T)csup,
+                           near, rm);
     hipLaunchKernelGGL((k_fluid_final<T, INTER, STEP>), dim3(blocks_for(gs->n, bt)), dim3(bt), 0,
-                       st, prm<T>(p), view<T>(P), view<T>(D), view<T>(gs), view<T>(gd),
-                       (T)csup, near, tuned_remap(SPHDEM_PASS_FLUID_FINAL));
+                       st, prm<T>(p), view<T>(P), (const int *)gs->rank, gs->n, near, rm);
 }
 
 template <typename T>

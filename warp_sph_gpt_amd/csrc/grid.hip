@@ -10,7 +10,8 @@
 //   3. scatter: point_ids[cell_lo[key] + rank[i]] = i
 //   4. order:   each bucket's ids sorted ascending, which makes the order
 //               equal to a stable sort by key and independent of the order
-//               the atomics were served in
+//               the atomics were served in; rank[i] becomes i's sorted slot
+//               (the inverse of point_ids)
 //   5. sorted:  sorted[k] = (p[id], w[id]) for coalesced neighbour reads
 #include "common.hpp"
 
@@ -292,9 +293,10 @@ __global__ void k_grid_scatter(const int *cells, const int *rank, const int *lo,
 
 // Warp's order inside a bucket is ascending point index.  The scatter left
 // each bucket's ids in atomic-arrival order in `tmp`; point i takes the slot
-// after the ids of its bucket smaller than i.
+// after the ids of its bucket smaller than i, and records it in slot[i] (the
+// arrival ranks there are no longer needed).
 __global__ void k_grid_order(const int *cells, const int *lo, const int *hi, const int *tmp,
-                             int n, int *ids) {
+                             int n, int *ids, int *slot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int c = cells[i], a = lo[c], e = hi[c];
@@ -309,6 +311,7 @@ __global__ void k_grid_order(const int *cells, const int *lo, const int *hi, con
     }
     for (; k < e; ++k) pos += tmp[k] < i;
     ids[pos] = i;
+    slot[i] = pos;
 }
 
 template <typename T>
@@ -345,7 +348,8 @@ static int grid_build(sphdem_grid_t *g, const void *points, int n, double radius
         hipLaunchKernelGGL(k_grid_scatter, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
                            g->point_cells, g->rank, g->cell_lo, n, tmp);
         hipLaunchKernelGGL(k_grid_order, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
-                           g->point_cells, g->cell_lo, g->cell_hi, tmp, n, g->point_ids);
+                           g->point_cells, g->cell_lo, g->cell_hi, tmp, n, g->point_ids,
+                           g->rank);
     }
     if (n > 0)
         hipLaunchKernelGGL(k_grid_sorted<T>, dim3(blocks_for(n, bt)), dim3(bt), 0, st,
